@@ -1,0 +1,554 @@
+// C ABI of libmpdata_hip.so (declared in include/mpdata_hip.h): argument
+// checking, per-domain scratch, and the sequencing of the per-stage kernels
+// and the fused step kernel on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpdata_hip.h"
+#include "mpdata_fused.hpp"
+#include "mpdata_stages.hpp"
+
+using namespace mpdata;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where) {
+    return fail(MPDATA_EHIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+#define HIPCHK(call)                                   \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+}  // namespace
+
+struct mpdata_ctx {
+    Geo g;
+    int real_bytes;
+    int path;
+    int device;
+    // scratch, allocated on first use
+    void* xant = nullptr;
+    void* psi2 = nullptr;
+    void* v[3] = {nullptr, nullptr, nullptr};
+    void* cp = nullptr;
+    void* cn = nullptr;
+    void* fns[3] = {nullptr, nullptr, nullptr};
+    void* f[3] = {nullptr, nullptr, nullptr};
+    void* ul[3] = {nullptr, nullptr, nullptr};
+    void* adv[3] = {nullptr, nullptr, nullptr};
+    void* red = nullptr;        // reduction partials (device)
+    long long red_bytes = 0;
+    long long held = 0;         // bytes of device scratch
+    std::vector<void*> owned;
+};
+
+namespace {
+
+long long field_len(const Geo& g, int kind) {
+    switch (kind) {
+        case FACEX: return len_of<FACEX>(g);
+        case FACEY: return len_of<FACEY>(g);
+        case FACEZ: return len_of<FACEZ>(g);
+        default: return len_of<SCAL>(g);
+    }
+}
+
+int ensure(mpdata_ctx* c, void** slot, int kind) {
+    if (*slot) return MPDATA_OK;
+    size_t bytes = (size_t)field_len(c->g, kind) * c->real_bytes;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MPDATA_ENOMEM, "hipMalloc(%zu bytes) for scratch failed: %s", bytes,
+                    hipGetErrorString(e));
+    }
+    // zero once so that halo cells never read are deterministic
+    e = hipMemset(p, 0, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMemset(scratch)");
+    *slot = p;
+    c->owned.push_back(p);
+    c->held += (long long)bytes;
+    return MPDATA_OK;
+}
+
+int ensure_red(mpdata_ctx* c, long long bytes) {
+    if (c->red_bytes >= bytes) return MPDATA_OK;
+    if (c->red) {
+        (void)hipFree(c->red);
+        c->held -= c->red_bytes;
+        c->red = nullptr;
+        c->red_bytes = 0;
+    }
+    hipError_t e = hipMalloc(&c->red, (size_t)bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MPDATA_ENOMEM, "hipMalloc(%lld bytes) for reductions failed", bytes);
+    }
+    c->red_bytes = bytes;
+    c->held += bytes;
+    return MPDATA_OK;
+}
+
+int check_ctx(const mpdata_ctx* c, int real_bytes) {
+    if (!c) return fail(MPDATA_EINVAL, "null context");
+    if (c->real_bytes != real_bytes)
+        return fail(MPDATA_EINVAL, "context holds %d-byte reals, call needs %d", c->real_bytes,
+                    real_bytes);
+    int dev = -1;
+    if (hipGetDevice(&dev) == hipSuccess && dev != c->device)
+        return fail(MPDATA_EINVAL, "context was created on device %d, current device is %d",
+                    c->device, dev);
+    return MPDATA_OK;
+}
+
+int check_dims(const mpdata_ctx* c, int ipoles, int ibcx, int ibcy, int ibcz, int np, int mp,
+               int lp, int ih) {
+    if (ipoles != 0)
+        return fail(MPDATA_EUNSUPPORTED, "ipoles0=%d: only the Cartesian case (0) exists", ipoles);
+    const Geo& g = c->g;
+    if (np != g.np || mp != g.mp || lp != g.lp || ih != g.ih)
+        return fail(MPDATA_EINVAL, "shape (%d,%d,%d,ih=%d) differs from the context (%d,%d,%d,ih=%d)",
+                    np, mp, lp, ih, g.np, g.mp, g.lp, g.ih);
+    if (ibcx != g.px || ibcy != g.py || ibcz != g.pz)
+        return fail(MPDATA_EINVAL, "boundary flags (%d,%d,%d) differ from the context (%d,%d,%d)",
+                    ibcx, ibcy, ibcz, g.px, g.py, g.pz);
+    return MPDATA_OK;
+}
+
+template <typename... P>
+bool any_null(P... p) {
+    const void* a[] = {(const void*)p...};
+    for (const void* q : a)
+        if (!q) return true;
+    return false;
+}
+
+// One corrective pass on the per-stage kernels (see orc_antidiff_pass of the
+// oracle for the sequence; antidiff3d-gauge-gpubc.f90:71-430,
+// antidiff3d-standard-gpubc.f90:59-462).
+template <typename T>
+int stage_pass(mpdata_ctx* c, int variant, int nonos, const T* u1, const T* u2, const T* u3,
+               T* psi, T* x0, T* out, const T* rhr, const T* h, T* ul1, T* ul2, T* ul3,
+               hipStream_t s) {
+    using St = Stages<T>;
+    const Geo& g = c->g;
+    const bool stdv = variant == MPDATA_STANDARD;
+    int rc;
+    for (int d = 0; d < 3; d++)
+        if ((rc = ensure(c, &c->v[d], FACEX + d))) return rc;
+    if (nonos) {
+        if ((rc = ensure(c, &c->cp, SCAL))) return rc;
+        if ((rc = ensure(c, &c->cn, SCAL))) return rc;
+    }
+    if (stdv)
+        for (int d = 0; d < 3; d++) {
+            if ((rc = ensure(c, &c->fns[d], FACEX + d))) return rc;
+            if ((rc = ensure(c, &c->f[d], FACEX + d))) return rc;
+        }
+    T *v1 = (T*)c->v[0], *v2 = (T*)c->v[1], *v3 = (T*)c->v[2];
+    T *f1ns = (T*)c->fns[0], *f2ns = (T*)c->fns[1], *f3ns = (T*)c->fns[2];
+    T *cp = (T*)c->cp, *cn = (T*)c->cn;
+    HIPCHK(St::halo_xy(g, psi, St::MIRROR, 1, s));
+    HIPCHK(St::pseudo_vel(g, variant, u1, u2, u3, psi, h, v1, v2, v3, f1ns, f2ns, f3ns, s));
+    HIPCHK(St::face_bc(g, v1, v2, v3, s));
+    if (stdv) HIPCHK(St::face_bc(g, f1ns, f2ns, f3ns, s));
+    HIPCHK(St::halo_xy(g, x0, St::COPY, 0, s));
+    if (nonos) {
+        HIPCHK(St::limiter(g, x0, psi, h, stdv ? f1ns : v1, stdv ? f2ns : v2, stdv ? f3ns : v3,
+                           cp, cn, s));
+        HIPCHK(St::halo_xy(g, cp, St::ZERO, 1, s));
+        HIPCHK(St::halo_xy(g, cn, St::ZERO, 1, s));
+    }
+    if (!stdv) {
+        HIPCHK(St::gauge_update(g, nonos, v1, v2, v3, cp, cn, psi, rhr, h, out, ul1, ul2, ul3, s));
+        return MPDATA_OK;
+    }
+    T *f1 = (T*)c->f[0], *f2 = (T*)c->f[1], *f3 = (T*)c->f[2];
+    HIPCHK(St::std_flux(g, nonos, v1, v2, v3, cp, cn, psi, rhr, f1, f2, f3, ul1, ul2, ul3, s));
+    HIPCHK(St::face_bc(g, f1, f2, f3, s));
+    if (ul1) HIPCHK(St::face_bc(g, ul1, ul2, ul3, s));
+    HIPCHK(St::std_update(g, f1, f2, f3, psi, rhr, h, out, s));
+    return MPDATA_OK;
+}
+
+template <typename T>
+int upwind_impl(mpdata_ctx* c, const T* u1, const T* u2, const T* u3, T* x, T* xant,
+                const T* rhr, const T* h, const T* bcx, const T* bcy, hipStream_t s) {
+    const Geo& g = c->g;
+    if ((!g.px && !bcx) || (!g.py && !bcy))
+        return fail(MPDATA_EINVAL, "open boundaries need bcx/bcy");
+    HIPCHK(Stages<T>::putbcinxh(g, x, bcx, bcy, s));
+    HIPCHK(Stages<T>::upwind(g, u1, u2, u3, x, xant, rhr, h, s));
+    return MPDATA_OK;
+}
+
+template <typename T>
+int step_stages(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T* u1,
+                const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+                hipStream_t s) {
+    int rc;
+    if ((rc = ensure(c, &c->xant, SCAL))) return rc;
+    T* xant = (T*)c->xant;
+    if ((rc = upwind_impl<T>(c, u1, u2, u3, x, xant, rhr, h, bcx, bcy, s))) return rc;
+    if (variant == MPDATA_UPWIND || iters <= 1) {
+        HIPCHK(Stages<T>::copy_interior(c->g, xant, x, s));
+        return MPDATA_OK;
+    }
+    if (iters > 2) {
+        if ((rc = ensure(c, &c->psi2, SCAL))) return rc;
+        for (int d = 0; d < 3; d++) {
+            if ((rc = ensure(c, &c->ul[d], FACEX + d))) return rc;
+            if ((rc = ensure(c, &c->adv[d], FACEX + d))) return rc;
+        }
+    }
+    T* psi = xant;
+    T* nxt = (T*)c->psi2;
+    T* adv[3] = {(T*)c->adv[0], (T*)c->adv[1], (T*)c->adv[2]};
+    T* lim[3] = {(T*)c->ul[0], (T*)c->ul[1], (T*)c->ul[2]};
+    for (int it = 2; it <= iters; it++) {
+        const bool last = it == iters;
+        const T* a1 = it == 2 ? u1 : adv[0];
+        const T* a2 = it == 2 ? u2 : adv[1];
+        const T* a3 = it == 2 ? u3 : adv[2];
+        rc = stage_pass<T>(c, variant, nonos, a1, a2, a3, psi, x, last ? x : nxt, rhr, h,
+                           last ? nullptr : lim[0], last ? nullptr : lim[1],
+                           last ? nullptr : lim[2], s);
+        if (rc) return rc;
+        if (!last) {
+            for (int d = 0; d < 3; d++) {
+                T* t = adv[d];
+                adv[d] = lim[d];
+                lim[d] = t;
+            }
+            T* t = psi;
+            psi = nxt;
+            nxt = t;
+        }
+    }
+    return MPDATA_OK;
+}
+
+template <typename T>
+int step_impl(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T* u1, const T* u2,
+              const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+              hipStream_t s) {
+    if (variant < MPDATA_UPWIND || variant > MPDATA_STANDARD)
+        return fail(MPDATA_EINVAL, "variant %d", variant);
+    if (variant != MPDATA_UPWIND && (iters < 2 || iters > 8))
+        return fail(MPDATA_EINVAL, "iters %d outside 2..8", iters);
+    if (nonos != 0 && nonos != 1) return fail(MPDATA_EINVAL, "nonos %d", nonos);
+    if (any_null(x, u1, u2, u3, rhr, h)) return fail(MPDATA_EINVAL, "null field pointer");
+    const bool fusable = Fused<T>::supports(c->g, variant, iters, nonos);
+    if (c->path == MPDATA_PATH_FUSED && !fusable)
+        return fail(MPDATA_EUNSUPPORTED,
+                    "fused path supports variant %d iters %d nonos %d only for %s", variant,
+                    iters, nonos, Fused<T>::scope());
+    if (c->path != MPDATA_PATH_STAGES && fusable) {
+        if ((!c->g.px && !bcx) || (!c->g.py && !bcy))
+            return fail(MPDATA_EINVAL, "open boundaries need bcx/bcy");
+        HIPCHK(Fused<T>::step(c->g, x, u1, u2, u3, rhr, h, bcx, bcy, s));
+        return MPDATA_OK;
+    }
+    return step_stages<T>(c, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy, s);
+}
+
+template <typename T>
+int reduce_courants(mpdata_ctx* c, const T* ox, const T* oy, const T* oz, const T* h, int level,
+                    T dxi, T dyi, T dzi, T dt, double* out, hipStream_t s) {
+    if (any_null(ox, oy, oz, out) || (level == 2 && !h))
+        return fail(MPDATA_EINVAL, "null pointer");
+    if (level != 1 && level != 2)
+        return fail(MPDATA_EINVAL, "ntimelevel %d (reference accepts 1 or 2)", level);
+    const long long nb = Stages<T>::reduce_blocks(c->g);
+    int rc;
+    if ((rc = ensure_red(c, nb * 7 * (long long)sizeof(T)))) return rc;
+    T* part = (T*)c->red;
+    HIPCHK(Stages<T>::courants(c->g, ox, oy, oz, h, level, dt * dxi, dt * dyi, dt * dzi, part, s));
+    std::vector<T> hpart((size_t)nb * 7);
+    HIPCHK(hipMemcpyAsync(hpart.data(), part, hpart.size() * sizeof(T), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int q = 0; q < 7; q++) {
+        T m = T(0);
+        for (long long b = 0; b < nb; b++) m = hpart[7 * b + q] > m ? hpart[7 * b + q] : m;
+        out[q] = (double)m;
+    }
+    return MPDATA_OK;
+}
+
+template <typename T>
+int reduce_norms(mpdata_ctx* c, const T* x, const T* x0, double* out, hipStream_t s) {
+    if (any_null(x, out)) return fail(MPDATA_EINVAL, "null pointer");
+    const long long nb = Stages<T>::reduce_blocks(c->g);
+    int rc;
+    if ((rc = ensure_red(c, nb * 4 * (long long)sizeof(double)))) return rc;
+    double* part = (double*)c->red;
+    HIPCHK(Stages<T>::norm_sums(c->g, x, x0, part, s));
+    std::vector<double> hpart((size_t)nb * 4);
+    HIPCHK(hipMemcpyAsync(hpart.data(), part, hpart.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int q = 0; q < 4; q++) {
+        double acc = 0.0;
+        for (long long b = 0; b < nb; b++) acc += hpart[4 * b + q];
+        out[q] = acc;
+    }
+    return MPDATA_OK;
+}
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char* mpdata_version(void) { return "mpdata_hip 0.1 (gfx950)"; }
+
+const char* mpdata_last_error(void) { return g_last_error.c_str(); }
+
+int mpdata_ctx_create(int np, int mp, int lp, int ih, int ibcx, int ibcy, int ibcz,
+                      int real_bytes, mpdata_ctx** out) {
+    if (!out) return fail(MPDATA_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (np < 3 || mp < 3 || lp < 3) return fail(MPDATA_EINVAL, "np,mp,lp must be >= 3");
+    if (ih < 1) return fail(MPDATA_EINVAL, "ih must be >= 1");
+    if ((ibcx | ibcy | ibcz) & ~1) return fail(MPDATA_EINVAL, "ibc flags must be 0 or 1");
+    if (real_bytes != 4 && real_bytes != 8) return fail(MPDATA_EINVAL, "real_bytes must be 4 or 8");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    mpdata_ctx* c = new mpdata_ctx();
+    c->g = Geo{np, mp, lp, ih, ibcx, ibcy, ibcz};
+    c->real_bytes = real_bytes;
+    c->path = MPDATA_PATH_AUTO;
+    c->device = dev;
+    *out = c;
+    return MPDATA_OK;
+}
+
+int mpdata_ctx_destroy(mpdata_ctx* ctx) {
+    if (!ctx) return MPDATA_OK;
+    for (void* p : ctx->owned) (void)hipFree(p);
+    if (ctx->red) (void)hipFree(ctx->red);
+    delete ctx;
+    return MPDATA_OK;
+}
+
+int mpdata_ctx_set_path(mpdata_ctx* ctx, int path) {
+    if (!ctx) return fail(MPDATA_EINVAL, "null context");
+    if (path < MPDATA_PATH_AUTO || path > MPDATA_PATH_FUSED) return fail(MPDATA_EINVAL, "path %d", path);
+    ctx->path = path;
+    return MPDATA_OK;
+}
+
+long long mpdata_ctx_scratch_bytes(const mpdata_ctx* ctx) { return ctx ? ctx->held : 0; }
+
+#define DEFINE_REAL_API(T, SFX, RB)                                                              \
+    int mpdata_upwind3d_gpubc##SFX(mpdata_ctx* ctx, const T* u1, const T* u2, const T* u3,       \
+                                   T* x_in, T* xant, const T* xforc_impl, const T* xforc_expl,   \
+                                   const T* rhr, const T* h, const T* bcx, const T* bcy,         \
+                                   int ipoles0, int ibcx0, int ibcy0, int ibcz0, int np, int mp, \
+                                   int lp, int ih, T dt, void* stream) {                         \
+        (void)xforc_impl; (void)xforc_expl; (void)dt;                                            \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        if ((rc = check_dims(ctx, ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih))) return rc;     \
+        if (any_null(u1, u2, u3, x_in, xant, rhr, h)) return fail(MPDATA_EINVAL, "null field"); \
+        return upwind_impl<T>(ctx, u1, u2, u3, x_in, xant, rhr, h, bcx, bcy,                     \
+                              (hipStream_t)stream);                                              \
+    }                                                                                            \
+    static int antidiff##SFX(int variant, mpdata_ctx* ctx, const T* u1, const T* u2,             \
+                             const T* u3, T* xant, T* x_in, const T* rhr, const T* h,            \
+                             int ipoles0, int ibcx0, int ibcy0, int ibcz0, int np, int mp,       \
+                             int lp, int ih, void* stream) {                                     \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        if ((rc = check_dims(ctx, ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih))) return rc;     \
+        if (any_null(u1, u2, u3, x_in, xant, rhr, h)) return fail(MPDATA_EINVAL, "null field"); \
+        return stage_pass<T>(ctx, variant, 1, u1, u2, u3, xant, x_in, x_in, rhr, h, nullptr,     \
+                             nullptr, nullptr, (hipStream_t)stream);                             \
+    }                                                                                            \
+    int mpdata_antidiff3d_gauge_gpubc##SFX(mpdata_ctx* ctx, int lupdatemulti, const T* u1,       \
+                                           const T* u2, const T* u3, T* xant, T* x_in,           \
+                                           const T* rhr, const T* h, int iflip, int ipoles0,     \
+                                           int ibcx0, int ibcy0, int ibcz0, int np, int mp,      \
+                                           int lp, int ih, void* stream) {                       \
+        (void)lupdatemulti; (void)iflip;                                                         \
+        return antidiff##SFX(MPDATA_GAUGE, ctx, u1, u2, u3, xant, x_in, rhr, h, ipoles0, ibcx0,  \
+                             ibcy0, ibcz0, np, mp, lp, ih, stream);                              \
+    }                                                                                            \
+    int mpdata_antidiff3d_standard_gpubc##SFX(mpdata_ctx* ctx, int lupdatemulti, const T* u1,    \
+                                              const T* u2, const T* u3, T* xant, T* x_in,        \
+                                              const T* rhr, const T* h, int iflip, int ipoles0,  \
+                                              int ibcx0, int ibcy0, int ibcz0, int np, int mp,   \
+                                              int lp, int ih, void* stream) {                    \
+        (void)lupdatemulti; (void)iflip;                                                         \
+        return antidiff##SFX(MPDATA_STANDARD, ctx, u1, u2, u3, xant, x_in, rhr, h, ipoles0,      \
+                             ibcx0, ibcy0, ibcz0, np, mp, lp, ih, stream);                       \
+    }                                                                                            \
+    int mpdata_advec_dwarf##SFX(mpdata_ctx* ctx, int legacyoptmode, int t_adv,                   \
+                                int lupdatemulti, int lvertsplit, int np, int mp, int lp,        \
+                                int ih, int ipoles, int ibcx, int ibcy, int ibcz, T* x,          \
+                                const T* x_lr, const T* x_bt, const T* uadv, const T* vadv,      \
+                                const T* wadv, const T* rhr, const T* rhoadv, void* stream) {    \
+        (void)lupdatemulti;                                                                      \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        if ((rc = check_dims(ctx, ipoles, ibcx, ibcy, ibcz, np, mp, lp, ih))) return rc;         \
+        if (lvertsplit) return fail(MPDATA_EUNSUPPORTED, "lvertsplit (commented out in advec.f90)"); \
+        int variant;                                                                             \
+        if (legacyoptmode == 6)                                                                  \
+            variant = MPDATA_UPWIND;                                                             \
+        else if (legacyoptmode == 4 && (t_adv == 1 || t_adv == 2))                               \
+            variant = t_adv == 1 ? MPDATA_GAUGE : MPDATA_STANDARD;                               \
+        else                                                                                     \
+            return fail(MPDATA_EINVAL,                                                           \
+                        "legacyoptmode %d / t_adv %d: advec.f90 accepts 4 with t_adv 1|2, or 6", \
+                        legacyoptmode, t_adv);                                                   \
+        return step_impl<T>(ctx, variant, 2, 1, x, uadv, vadv, wadv, rhr, rhoadv, x_lr, x_bt,    \
+                            (hipStream_t)stream);                                                \
+    }                                                                                            \
+    int mpdata_step##SFX(mpdata_ctx* ctx, int variant, int iters, int nonos, T* x, const T* u1,  \
+                         const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,       \
+                         const T* bcy, void* stream) {                                           \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        return step_impl<T>(ctx, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy,         \
+                            (hipStream_t)stream);                                                \
+    }                                                                                            \
+    int mpdata_c_timeloop##SFX(mpdata_ctx* ctx, int variant, int iters, int nonos, int nt, T* x, \
+                               const T* u1, const T* u2, const T* u3, const T* rhr, const T* h,  \
+                               const T* bcx, const T* bcy, void* stream) {                       \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        if (nt < 0) return fail(MPDATA_EINVAL, "nt %d", nt);                                     \
+        for (int it = 0; it < nt; it++)                                                          \
+            if ((rc = step_impl<T>(ctx, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy,  \
+                                   (hipStream_t)stream)))                                        \
+                return rc;                                                                       \
+        return MPDATA_OK;                                                                        \
+    }                                                                                            \
+    int mpdata_velprd_traj0##SFX(mpdata_ctx* ctx, T* ox, T* oy, T* oz, T* u1, T* u2, T* u3,      \
+                                 const T* h, T dxi, T dyi, T dzi, T dt, T dt_ratio,              \
+                                 void* stream) {                                                 \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        if (any_null(ox, oy, oz, u1, u2, u3, h)) return fail(MPDATA_EINVAL, "null field");      \
+        const Geo& g = ctx->g;                                                                   \
+        const long long n = len_of<SCAL>(g);                                                     \
+        hipStream_t s = (hipStream_t)stream;                                                     \
+        HIPCHK(Stages<T>::velprd_extrap(g, ox, oy, oz, h, dt * dxi, dt * dyi, dt * dzi, dt_ratio, \
+                                        s));                                                     \
+        HIPCHK(Stages<T>::a_to_c(g, ox + n, oy + n, oz + n, u1, u2, u3, s));                     \
+        return MPDATA_OK;                                                                        \
+    }                                                                                            \
+    int mpdata_velprdA_to_C##SFX(mpdata_ctx* ctx, T* ox, T* oy, T* oz, T* u1, T* u2, T* u3,      \
+                                 void* stream) {                                                 \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        if (any_null(ox, oy, oz, u1, u2, u3)) return fail(MPDATA_EINVAL, "null field");         \
+        HIPCHK(Stages<T>::a_to_c(ctx->g, ox, oy, oz, u1, u2, u3, (hipStream_t)stream));          \
+        return MPDATA_OK;                                                                        \
+    }                                                                                            \
+    int mpdata_velprd_rotate##SFX(mpdata_ctx* ctx, T* u, T* v, T* w, T* ox, T* oy, T* oz,        \
+                                  const T* h, T dt_ratio, void* stream) {                        \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        if (any_null(u, v, w, ox, oy, oz, h)) return fail(MPDATA_EINVAL, "null field");         \
+        HIPCHK(Stages<T>::velprd_rotate(ctx->g, u, v, w, ox, oy, oz, h, dt_ratio,                \
+                                        (hipStream_t)stream));                                   \
+        return MPDATA_OK;                                                                        \
+    }                                                                                            \
+    int mpdata_compute_courants##SFX(mpdata_ctx* ctx, const T* ox, const T* oy, const T* oz,     \
+                                     const T* h, int ntimelevel, T dxi, T dyi, T dzi, T dt,      \
+                                     double* out, void* stream) {                                \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        return reduce_courants<T>(ctx, ox, oy, oz, h, ntimelevel, dxi, dyi, dzi, dt, out,        \
+                                  (hipStream_t)stream);                                          \
+    }                                                                                            \
+    int mpdata_norm_sums##SFX(mpdata_ctx* ctx, const T* x, const T* x0, double* out,             \
+                              void* stream) {                                                    \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        return reduce_norms<T>(ctx, x, x0, out, (hipStream_t)stream);                            \
+    }
+
+DEFINE_REAL_API(double, _f64, 8)
+DEFINE_REAL_API(float, _f32, 4)
+
+int upwind_c_timeloop(void* xtracer, void* uadv, void* vadv, void* wadv, void* rhr, void* h,
+                      int np, int mp, int lp, int ih, int nt) {
+    if (any_null(xtracer, uadv, vadv, wadv, rhr, h)) return fail(MPDATA_EINVAL, "null pointer");
+    if (nt < 0) return fail(MPDATA_EINVAL, "nt %d", nt);
+    mpdata_ctx* c = nullptr;
+    int rc = mpdata_ctx_create(np, mp, lp, ih, 0, 0, 0, 8, &c);
+    if (rc) return rc;
+    const Geo g = c->g;
+    const size_t sizes[6] = {(size_t)len_of<SCAL>(g), (size_t)len_of<FACEX>(g),
+                             (size_t)len_of<FACEY>(g), (size_t)len_of<FACEZ>(g),
+                             (size_t)len_of<SCAL>(g), (size_t)len_of<SCAL>(g)};
+    void* user[6] = {xtracer, uadv, vadv, wadv, rhr, h};
+    void* dev[6] = {nullptr};
+    bool staged[6] = {false};
+    double* bc = nullptr;
+    const size_t nbc = 2 * (size_t)mp * lp + 2 * (size_t)np * lp;
+    hipError_t e = hipSuccess;
+    for (int a = 0; a < 6 && e == hipSuccess; a++) {
+        hipPointerAttribute_t attr;
+        bool on_device = hipPointerGetAttributes(&attr, user[a]) == hipSuccess &&
+                         (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged);
+        (void)hipGetLastError();
+        if (on_device) {
+            dev[a] = user[a];
+            continue;
+        }
+        e = hipMalloc(&dev[a], sizes[a] * sizeof(double));
+        staged[a] = e == hipSuccess;
+        if (e == hipSuccess)
+            e = hipMemcpy(dev[a], user[a], sizes[a] * sizeof(double), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMalloc(&bc, nbc * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(bc, 0, nbc * sizeof(double));
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "upwind_c_timeloop staging");
+    } else {
+        for (int it = 0; it < nt && rc == MPDATA_OK; it++)
+            rc = step_impl<double>(c, MPDATA_UPWIND, 2, 1, (double*)dev[0], (double*)dev[1],
+                                   (double*)dev[2], (double*)dev[3], (double*)dev[4],
+                                   (double*)dev[5], bc, bc + 2 * (size_t)mp * lp, nullptr);
+        if (rc == MPDATA_OK && staged[0]) {
+            e = hipMemcpy(xtracer, dev[0], sizes[0] * sizeof(double), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = hip_fail(e, "upwind_c_timeloop copy-out");
+        }
+        if (rc == MPDATA_OK) {
+            e = hipDeviceSynchronize();
+            if (e != hipSuccess) rc = hip_fail(e, "upwind_c_timeloop");
+        }
+    }
+    for (int a = 0; a < 6; a++)
+        if (staged[a]) (void)hipFree(dev[a]);
+    if (bc) (void)hipFree(bc);
+    mpdata_ctx_destroy(c);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,19 @@
+// Fused single-pass MPDATA step (mpdata_fused.hip).
+#pragma once
+#include "mpdata_common.hpp"
+
+namespace mpdata {
+
+template <typename T>
+struct Fused {
+    // True when the fused kernel implements this configuration.
+    static bool supports(const Geo& g, int variant, int iters, int nonos);
+    // Human-readable scope of the fused kernel, for error messages.
+    static const char* scope();
+    // One full step x <- MPDATA(x); reads x (and bcx/bcy on open walls),
+    // writes the interior of x through a ping-pong buffer held per device.
+    static hipError_t step(const Geo& g, T* x, const T* u1, const T* u2, const T* u3,
+                           const T* rhr, const T* h, const T* bcx, const T* bcy, hipStream_t s);
+};
+
+}  // namespace mpdata

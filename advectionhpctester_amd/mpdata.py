@@ -1,0 +1,237 @@
+"""Python host mirror of the reference's advection interface, on GPU tensors.
+
+``Domain`` owns one libmpdata_hip context (the per-domain scratch that the
+reference keeps in module globals, common/scratch_datafields_modular.f90)
+and exposes the reference's subroutines with their argument lists:
+
+* ``upwind3d_gpubc``            advection/src_algorithms/upwind3d-gpubc.f90:20-28
+* ``antidiff3d_gauge_gpubc``    advection/src_algorithms/antidiff3d-gauge-gpubc.f90:14-16
+* ``antidiff3d_standard_gpubc`` advection/src_algorithms/antidiff3d-standard-gpubc.f90:14-16
+* ``advec_dwarf``               advection/src_drivers/advec.f90:116-136
+* ``velprd_traj0`` / ``velprdA_to_C`` / ``velprd_rotate``
+                                advection/src_algorithms/module_velprd.f90:15-364,
+                                velprd_driver.f90:254-270
+* ``compute_courants``          common/diagutils.f90:588-672
+
+plus ``step``/``timeloop`` for the generalised step (iterations, limiter
+switch).  Fields are torch tensors on the GPU with the reference's memory
+layout: shape (lp+2ih, mp+2ih, np+2ih) in C order is the Fortran array
+(1-ih:np+ih, 1-ih:mp+ih, 1-ih:lp+ih); face arrays add one along their normal
+(see :func:`field_shape`).  Errors raise :class:`MpdataError` with the
+library's message, as the reference's STOPs end the run.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from ._lib import LIB, MpdataError, check  # noqa: F401  (re-exported)
+
+UPWIND, GAUGE, STANDARD = _lib.UPWIND, _lib.GAUGE, _lib.STANDARD
+PATH_AUTO, PATH_STAGES, PATH_FUSED = _lib.PATH_AUTO, _lib.PATH_STAGES, _lib.PATH_FUSED
+
+
+def field_shape(np_, mp, lp, ih=1, kind=0):
+    """Tensor shape of a field: kind 0 scalar, 1/2/3 faces normal to x/y/z."""
+    return (lp + 2 * ih + (kind == 3), mp + 2 * ih + (kind == 2), np_ + 2 * ih + (kind == 1))
+
+
+def interior(a, ih=1):
+    """View of the np x mp x lp interior of a scalar field tensor."""
+    return a[ih:a.shape[0] - ih, ih:a.shape[1] - ih, ih:a.shape[2] - ih]
+
+
+class Domain:
+    """One grid of np x mp x lp cells (halo ih) on the current GPU."""
+
+    def __init__(self, np_, mp, lp, ih=1, ibc=(0, 0, 0), dtype=torch.float64, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("advectionhpctester_amd needs a GPU (HIP device); none is visible")
+        self.np, self.mp, self.lp, self.ih = int(np_), int(mp), int(lp), int(ih)
+        self.ibc = tuple(int(b) for b in ibc)
+        self.dtype = dtype
+        if dtype == torch.float64:
+            self.sfx, self.real, rb = "_f64", ctypes.c_double, 8
+        elif dtype == torch.float32:
+            self.sfx, self.real, rb = "_f32", ctypes.c_float, 4
+        else:
+            raise TypeError(f"dtype {dtype}: float64 or float32")
+        self.device = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        self._ctx = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(LIB.mpdata_ctx_create(self.np, self.mp, self.lp, self.ih, *self.ibc, rb,
+                                        ctypes.byref(self._ctx)))
+
+    # -- plumbing -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None) and self._ctx.value:
+            LIB.mpdata_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def dims(self):
+        return (self.np, self.mp, self.lp, self.ih)
+
+    def shape(self, kind=0):
+        return field_shape(self.np, self.mp, self.lp, self.ih, kind)
+
+    def zeros(self, kind=0):
+        return torch.zeros(self.shape(kind), dtype=self.dtype, device=self.device)
+
+    def set_path(self, path):
+        check(LIB.mpdata_ctx_set_path(self._ctx, int(path)))
+
+    def scratch_bytes(self):
+        return int(LIB.mpdata_ctx_scratch_bytes(self._ctx))
+
+    def _fn(self, name):
+        return getattr(LIB, name + self.sfx)
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _p(self, t, kind=None, name="field", allow_none=False):
+        """Device pointer of a tensor after checking dtype/device/shape."""
+        if t is None:
+            if allow_none:
+                return None
+            raise MpdataError(_lib.MPDATA_EINVAL, f"{name} is None")
+        if not isinstance(t, torch.Tensor):
+            raise TypeError(f"{name}: torch tensor expected")
+        if t.dtype != self.dtype:
+            raise MpdataError(_lib.MPDATA_EINVAL, f"{name}: dtype {t.dtype}, domain uses {self.dtype}")
+        if t.device != self.device:
+            raise MpdataError(_lib.MPDATA_EINVAL, f"{name}: on {t.device}, domain on {self.device}")
+        if not t.is_contiguous():
+            raise MpdataError(_lib.MPDATA_EINVAL, f"{name}: not contiguous")
+        if kind is not None:
+            want = self.shape(kind) if kind in (0, 1, 2, 3) else kind
+            if tuple(t.shape) != tuple(want):
+                raise MpdataError(_lib.MPDATA_EINVAL, f"{name}: shape {tuple(t.shape)}, expected {want}")
+        return ctypes.c_void_p(t.data_ptr())
+
+    def _bc(self, bcx, bcy):
+        px = self._p(bcx, (2, self.lp, self.mp), "bcx", allow_none=self.ibc[0] == 1)
+        py = self._p(bcy, (2, self.lp, self.np), "bcy", allow_none=self.ibc[1] == 1)
+        return px, py
+
+    def _fields(self, u1, u2, u3, rhr, h):
+        return (self._p(u1, 1, "u1"), self._p(u2, 2, "u2"), self._p(u3, 3, "u3"),
+                self._p(rhr, 0, "rhr"), self._p(h, 0, "h"))
+
+    # -- the reference's subroutines -----------------------------------------
+    def upwind3d_gpubc(self, u1, u2, u3, x_in, xant, xforc_impl, xforc_expl, rhr, h, bcx, bcy,
+                       ipoles0, ibcx0, ibcy0, ibcz0, np_, mp, lp, ih, dt):
+        pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
+        pbx, pby = self._bc(bcx, bcy)
+        check(self._fn("mpdata_upwind3d_gpubc")(
+            self._ctx, pu1, pu2, pu3, self._p(x_in, 0, "x_in"), self._p(xant, 0, "xant"),
+            self._p(xforc_impl, 0, "xforc_impl", True), self._p(xforc_expl, 0, "xforc_expl", True),
+            prhr, ph, pbx, pby, ipoles0, ibcx0, ibcy0, ibcz0, np_, mp, lp, ih, self.real(dt),
+            self._stream()))
+
+    def _antidiff(self, name, lupdatemulti, u1, u2, u3, xant, x_in, rhr, h, iflip, ipoles0,
+                  ibcx0, ibcy0, ibcz0, np_, mp, lp, ih):
+        pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
+        check(self._fn(name)(self._ctx, int(bool(lupdatemulti)), pu1, pu2, pu3,
+                             self._p(xant, 0, "xant"), self._p(x_in, 0, "x_in"), prhr, ph, iflip,
+                             ipoles0, ibcx0, ibcy0, ibcz0, np_, mp, lp, ih, self._stream()))
+
+    def antidiff3d_gauge_gpubc(self, lupdatemulti, u1, u2, u3, xant, x_in, rhr, h, iflip,
+                               ipoles0, ibcx0, ibcy0, ibcz0, np_, mp, lp, ih):
+        self._antidiff("mpdata_antidiff3d_gauge_gpubc", lupdatemulti, u1, u2, u3, xant, x_in,
+                       rhr, h, iflip, ipoles0, ibcx0, ibcy0, ibcz0, np_, mp, lp, ih)
+
+    def antidiff3d_standard_gpubc(self, lupdatemulti, u1, u2, u3, xant, x_in, rhr, h, iflip,
+                                  ipoles0, ibcx0, ibcy0, ibcz0, np_, mp, lp, ih):
+        self._antidiff("mpdata_antidiff3d_standard_gpubc", lupdatemulti, u1, u2, u3, xant, x_in,
+                       rhr, h, iflip, ipoles0, ibcx0, ibcy0, ibcz0, np_, mp, lp, ih)
+
+    def advec_dwarf(self, legacyoptmode, t_adv, lupdatemulti, lvertsplit, np_, mp, lp, ih,
+                    ipoles, ibcx, ibcy, ibcz, x, x_lr, x_bt, uadv, vadv, wadv, rhr, rhoadv):
+        pu1, pu2, pu3, prhr, ph = self._fields(uadv, vadv, wadv, rhr, rhoadv)
+        pbx, pby = self._bc(x_lr, x_bt)
+        check(self._fn("mpdata_advec_dwarf")(
+            self._ctx, legacyoptmode, t_adv, int(bool(lupdatemulti)), int(bool(lvertsplit)), np_,
+            mp, lp, ih, ipoles, ibcx, ibcy, ibcz, self._p(x, 0, "x"), pbx, pby, pu1, pu2, pu3,
+            prhr, ph, self._stream()))
+
+    # -- generalised step ------------------------------------------------------
+    def step(self, x, u1, u2, u3, rhr, h, bcx=None, bcy=None, variant=GAUGE, iters=2, nonos=1):
+        pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
+        pbx, pby = self._bc(bcx, bcy)
+        check(self._fn("mpdata_step")(self._ctx, variant, iters, nonos, self._p(x, 0, "x"), pu1,
+                                      pu2, pu3, prhr, ph, pbx, pby, self._stream()))
+
+    def timeloop(self, nt, x, u1, u2, u3, rhr, h, bcx=None, bcy=None, variant=GAUGE, iters=2,
+                 nonos=1):
+        pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
+        pbx, pby = self._bc(bcx, bcy)
+        check(self._fn("mpdata_c_timeloop")(self._ctx, variant, iters, nonos, nt,
+                                            self._p(x, 0, "x"), pu1, pu2, pu3, prhr, ph, pbx, pby,
+                                            self._stream()))
+
+    # -- velocity predictor ----------------------------------------------------
+    def _levels(self, t, name):
+        return self._p(t, (3,) + self.shape(0), name)
+
+    def velprd_traj0(self, ox, oy, oz, u1, u2, u3, h, dxi, dyi, dzi, dt, dt_ratio):
+        R = self.real
+        check(self._fn("mpdata_velprd_traj0")(
+            self._ctx, self._levels(ox, "ox"), self._levels(oy, "oy"), self._levels(oz, "oz"),
+            self._p(u1, 1, "u1"), self._p(u2, 2, "u2"), self._p(u3, 3, "u3"), self._p(h, 0, "h"),
+            R(dxi), R(dyi), R(dzi), R(dt), R(dt_ratio), self._stream()))
+
+    def velprdA_to_C(self, ox, oy, oz, u1, u2, u3):
+        check(self._fn("mpdata_velprdA_to_C")(
+            self._ctx, self._p(ox, 0, "ox"), self._p(oy, 0, "oy"), self._p(oz, 0, "oz"),
+            self._p(u1, 1, "u1"), self._p(u2, 2, "u2"), self._p(u3, 3, "u3"), self._stream()))
+
+    def velprd_rotate(self, u, v, w, ox, oy, oz, h, dt_ratio):
+        check(self._fn("mpdata_velprd_rotate")(
+            self._ctx, self._levels(u, "u"), self._levels(v, "v"), self._levels(w, "w"),
+            self._levels(ox, "ox"), self._levels(oy, "oy"), self._levels(oz, "oz"),
+            self._p(h, 0, "h"), self.real(dt_ratio), self._stream()))
+
+    # -- diagnostics -------------------------------------------------------------
+    def compute_courants(self, ox, oy, oz, h, ntimelevel, dxi, dyi, dzi, dt):
+        """(cr3d, crxy, crxz, cryz, crx, cry, crz); synchronous."""
+        R = self.real
+        out = (ctypes.c_double * 7)()
+        check(self._fn("mpdata_compute_courants")(
+            self._ctx, self._levels(ox, "ox"), self._levels(oy, "oy"), self._levels(oz, "oz"),
+            self._p(h, 0, "h", allow_none=ntimelevel == 1), ntimelevel, R(dxi), R(dyi), R(dzi),
+            R(dt), out, self._stream()))
+        return tuple(out)
+
+    def norm_sums(self, x, x0=None):
+        """(sum x^2, sum x0^2, sum (x0-x)^2, sum x) over the interior; synchronous."""
+        out = (ctypes.c_double * 4)()
+        check(self._fn("mpdata_norm_sums")(self._ctx, self._p(x, 0, "x"),
+                                           self._p(x0, 0, "x0", allow_none=True), out,
+                                           self._stream()))
+        return tuple(out)
+
+    def dwarf_norms(self, x, x0, timetot):
+        """Energy and Error L2 norms of the dwarf test (advection_test_setup.f90:472-492,
+        595-617) with nmli = 1/(n*m*l) (eulag_lib_parameters.f90:499)."""
+        s = self.norm_sums(x, x0)
+        nmli = 1.0 / (self.np * self.mp * self.lp)
+        energy = (abs(s[0] - s[1]) * nmli) ** 0.5 / timetot
+        error = (s[2] * nmli) ** 0.5 / timetot
+        return energy, error

@@ -1,0 +1,153 @@
+"""Initial conditions and synthetic flows of the advection tests (host, numpy).
+
+* :func:`dwarf_setup` is the rotating-sphere test of the reference dwarf
+  (advection/src_testapps/advection_dwarf_test_modular.f90:89-141 and
+  advection/src_testdata/advection_test_setup.f90:174-255, 619-637), with the
+  A-grid to C-grid averaging of module_velprd.f90:366-520 (open walls).
+* The other generators build the synthetic workloads the benchmarks and
+  parity tests use (solid-body rotation about a tilted axis, Fourier-mode
+  fields, deformational flow, slotted cylinders, Gaussian blobs).
+
+Arrays follow the field layout of :mod:`advectionhpctester_amd.mpdata`:
+shape (lp+2ih, mp+2ih, np+2ih) in C order, faces one longer along their
+normal.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+def shape(np_, mp, lp, ih=1, kind=0):
+    return (lp + 2 * ih + (kind == 3), mp + 2 * ih + (kind == 2), np_ + 2 * ih + (kind == 1))
+
+
+def a_to_c_open(ua, va, wa, ih=1, periodic=(0, 0, 0)):
+    """A-grid -> C-grid velocities: module_velprd.f90:166-364 (velprdA_to_C,
+    and velprdA_to_C_ver2 which coincides with it on open walls).
+
+    ua, va, wa are scalar-shaped arrays whose interiors hold the A-grid
+    velocity; their halos on open walls are overwritten with the one-sided
+    extrapolation a(0)=2a(1)-a(2), a(n+1)=2a(n)-a(n-1) (:222-264).  Periodic
+    directions average across the wrap (faces 1..n)."""
+    lp, mp, np_ = (s - 2 * ih for s in ua.shape)
+    I, J, K = slice(ih, ih + np_), slice(ih, ih + mp), slice(ih, ih + lp)
+    px, py, pz = periodic
+    if not px:
+        ua[K, J, ih - 1] = 2.0 * ua[K, J, ih] - ua[K, J, ih + 1]
+        ua[K, J, ih + np_] = 2.0 * ua[K, J, ih + np_ - 1] - ua[K, J, ih + np_ - 2]
+    if not py:
+        va[K, ih - 1, I] = 2.0 * va[K, ih, I] - va[K, ih + 1, I]
+        va[K, ih + mp, I] = 2.0 * va[K, ih + mp - 1, I] - va[K, ih + mp - 2, I]
+    if not pz:
+        wa[ih - 1, J, I] = 2.0 * wa[ih, J, I] - wa[ih + 1, J, I]
+        wa[ih + lp, J, I] = 2.0 * wa[ih + lp - 1, J, I] - wa[ih + lp - 2, J, I]
+    dt = ua.dtype
+    u1 = np.zeros(shape(np_, mp, lp, ih, 1), dt)
+    u2 = np.zeros(shape(np_, mp, lp, ih, 2), dt)
+    u3 = np.zeros(shape(np_, mp, lp, ih, 3), dt)
+    half = dt.type(0.5)
+    if px:
+        a = ua[K, J, I]
+        u1[K, J, I] = half * (a + np.roll(a, 1, axis=2))
+    else:
+        u1[K, J, ih:ih + np_ + 1] = half * (ua[K, J, ih:ih + np_ + 1] + ua[K, J, ih - 1:ih + np_])
+    if py:
+        a = va[K, J, I]
+        u2[K, J, I] = half * (a + np.roll(a, 1, axis=1))
+    else:
+        u2[K, ih:ih + mp + 1, I] = half * (va[K, ih:ih + mp + 1, I] + va[K, ih - 1:ih + mp, I])
+    if pz:
+        a = wa[K, J, I]
+        u3[K, J, I] = half * (a + np.roll(a, 1, axis=0))
+    else:
+        u3[ih:ih + lp + 1, J, I] = half * (wa[ih:ih + lp + 1, J, I] + wa[ih - 1:ih + lp, J, I])
+    return u1, u2, u3
+
+
+class DwarfCase:
+    """Everything the dwarf test needs: fields plus the norm constants."""
+
+    def __init__(self, n, ih, dx00, dt, nt, x, u1, u2, u3, h, rhr, bcx, bcy):
+        self.n, self.ih, self.dx00, self.dt, self.nt = n, ih, dx00, dt, nt
+        self.x, self.u1, self.u2, self.u3 = x, u1, u2, u3
+        self.h, self.rhr, self.bcx, self.bcy = h, rhr, bcx, bcy
+        self.x0 = x.copy()                      # xtest (advection_test_setup.f90:241)
+        self.timetot = nt * dt                  # eulag_lib_parameters.f90:510
+        self.nmli = 1.0 / (n * n * n)           # :499
+
+    def norms(self, x):
+        """(Energy L2, Error L2) of x against the initial field:
+        advection_test_setup.f90:472-492, 595-617."""
+        ih, n = self.ih, self.n
+        I = (slice(ih, ih + n),) * 3
+        a, e = x[I].astype(np.float64), self.x0[I].astype(np.float64)
+        energy = math.sqrt(abs((a * a).sum() - (e * e).sum()) * self.nmli) / self.timetot
+        error = math.sqrt(((e - a) ** 2).sum() * self.nmli) / self.timetot
+        return energy, error
+
+
+def dwarf_setup(iscale=1, ih=1, dtype=np.float64):
+    """The DWARF3 rotating-sphere case at n = 59*iscale
+    (advection_dwarf_test_modular.f90:92-141): dx00=100, dx=dx00/(n-1),
+    dt=0.018*2*pi/iscale, nt=556, sphere of radius 15 and value 4 on a zero
+    background, solid-body rotation with |omega|=0.1 about (1,1,1), h=rhr=1,
+    zero inflow (init_data, advection_test_setup.f90:619-637)."""
+    n = 59 * iscale
+    dx00 = 100.0
+    dx = dy = dz = dx00 / (n - 1)                     # eulag_lib_parameters.f90:504-506
+    dt = (0.018 * 2.0 * math.acos(-1.0)) / float(iscale)
+    nt = 556
+    # set_initial15 (advection_test_setup.f90:174-255)
+    sqrt3i = 1.0 / math.sqrt(3.0)
+    x0 = y0 = z0 = 0.5 * dx00
+    xc = x0 - 0.25 * dx00 * sqrt3i
+    yc = y0 + 0.25 * dx00 * sqrt3i
+    zc = z0 + 0.25 * dx00 * sqrt3i
+    r_squared = 15.0 * 15.0
+    h_sphere = 4.0
+    omega = 0.1 / math.sqrt(3.0)
+    omegax = omegay = omegaz = omega
+    idx = np.arange(1, n + 1, dtype=np.float64)
+    ka, ja, ia = np.meshgrid(idx, idx, idx, indexing="ij")
+    xp, yp, zp = ia * dx, ja * dy, ka * dz
+    tmp = (xp - xc) * (xp - xc) + (yp - yc) * (yp - yc) + (zp - zc) * (zp - zc)
+    S = shape(n, n, n, ih)
+    I = (slice(ih, ih + n),) * 3
+    x = np.zeros(S)
+    x[I] = np.where(tmp <= r_squared, h_sphere, 0.0)
+    ua, va, wa = np.zeros(S), np.zeros(S), np.zeros(S)
+    ua[I] = (-(omegaz * (ja * dy - y0)) + omegay * (ka * dz - z0)) * dt / dx
+    va[I] = (omegaz * (ia * dx - x0) - omegax * (ka * dz - z0)) * dt / dy
+    wa[I] = (-(omegay * (ia * dx - x0)) + omegax * (ja * dy - y0)) * dt / dz
+    u1, u2, u3 = a_to_c_open(ua, va, wa, ih)
+    h = np.ones(S)
+    rhr = np.ones(S)
+    bcx = np.zeros((2, n, n))
+    bcy = np.zeros((2, n, n))
+    cast = lambda a: a.astype(dtype)  # noqa: E731
+    return DwarfCase(n, ih, dx00, dt, nt, cast(x), cast(u1), cast(u2), cast(u3), cast(h),
+                     cast(rhr), cast(bcx), cast(bcy))
+
+
+def random_case(np_, mp, lp, ih=1, seed=0, dtype=np.float64, cmax=0.3, positive=True,
+                vary_h=True):
+    """Random fields for stage-level parity tests: x in [0.5,2] (or [-1,1]),
+    face velocities uniform in [-cmax,cmax], h and rhr in [0.8,1.2] (or 1),
+    random inflow values."""
+    rng = np.random.default_rng(seed)
+
+    def u(kind, lo, hi):
+        return rng.uniform(lo, hi, shape(np_, mp, lp, ih, kind)).astype(dtype)
+
+    x = u(0, 0.5, 2.0) if positive else u(0, -1.0, 1.0)
+    u1, u2, u3 = u(1, -cmax, cmax), u(2, -cmax, cmax), u(3, -cmax, cmax)
+    if vary_h:
+        h, rhr = u(0, 0.8, 1.2), u(0, 0.8, 1.2)
+    else:
+        h = np.ones(shape(np_, mp, lp, ih), dtype)
+        rhr = h.copy()
+    bcx = rng.uniform(0.5, 2.0, (2, lp, mp)).astype(dtype)
+    bcy = rng.uniform(0.5, 2.0, (2, lp, np_)).astype(dtype)
+    return dict(x=x, u1=u1, u2=u2, u3=u3, h=h, rhr=rhr, bcx=bcx, bcy=bcy)

@@ -1,0 +1,231 @@
+/*
+ * mpdata_hip.h -- C ABI of libmpdata_hip.so, the MI355X (gfx950) MPDATA
+ * advection step.
+ *
+ * Drop-in boundary for the reference's advection driver call
+ * (piotrows/advectionHPCtester, advection/src_drivers/advec.f90:116-244),
+ * whose seam is Fortran procedure-pointer dispatch between
+ * upwind3d_gpubc and antidiff3d_{gauge,standard}_gpubc (advec.f90:143-144).
+ * A Fortran host binds these entry points through iso_c_binding
+ * (advectionhpctester_amd/fortran/mpdata_hip.f90); Python binds them through
+ * ctypes (advectionhpctester_amd/_lib.py).
+ *
+ * Conventions
+ *  - Fields are the reference's Fortran arrays, column-major, lower bound
+ *    1-ih in every direction (i fastest):
+ *      scalar  x, xant, h(=rhoadv), rhr          (np+2ih)   (mp+2ih)   (lp+2ih)
+ *      x-faces u1(=uadv)                         (np+1+2ih) (mp+2ih)   (lp+2ih)
+ *      y-faces u2(=vadv)                         (np+2ih)   (mp+1+2ih) (lp+2ih)
+ *      z-faces u3(=wadv)                         (np+2ih)   (mp+2ih)   (lp+1+2ih)
+ *      bcx(mp,lp,2) = x_lr, bcy(np,lp,2) = x_bt  (no halo)
+ *  - Unless stated otherwise every pointer is GPU memory (hipMalloc or
+ *    managed) owned by the caller; the library owns only the scratch held
+ *    by an mpdata_ctx.
+ *  - 'stream' is a hipStream_t passed as void* (NULL = default stream).
+ *    Calls are asynchronous on that stream unless marked "synchronous".
+ *    A context is not re-entrant: calls sharing one context must be
+ *    ordered (one stream, or explicit events).
+ *  - Boundary flags ibcx/ibcy/ibcz: 0 = open (the reference's semantics),
+ *    1 = periodic, defined here as a clean period-n wrap in each direction
+ *    (the reference's single-process periodic path is incomplete).
+ *  - Every function returns MPDATA_OK (0) or an error code; the message of
+ *    the last error of the calling thread is mpdata_last_error().  Argument
+ *    checks happen before any GPU work, so invalid calls have no effect.
+ *  - _f64 functions need a context created with real_bytes = 8, _f32 with 4.
+ */
+#ifndef MPDATA_HIP_H
+#define MPDATA_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPDATA_OK 0
+#define MPDATA_EINVAL 1      /* invalid argument or shape mismatch        */
+#define MPDATA_EHIP 2        /* HIP runtime error                         */
+#define MPDATA_ENOMEM 3      /* device allocation failed                  */
+#define MPDATA_EUNSUPPORTED 4 /* option the reference has no meaning for  */
+
+/* variant codes (advec.f90:148-211: legacyoptmode 6 -> UPWIND,
+ * legacyoptmode 4 with t_adv 1 -> GAUGE, t_adv 2 -> STANDARD) */
+#define MPDATA_UPWIND 0
+#define MPDATA_GAUGE 1
+#define MPDATA_STANDARD 2
+
+/* execution paths of mpdata_step_* */
+#define MPDATA_PATH_AUTO 0   /* fused kernel when it supports the options */
+#define MPDATA_PATH_STAGES 1 /* one kernel per reference stage            */
+#define MPDATA_PATH_FUSED 2  /* single-pass fused kernel (error if unsupported) */
+
+typedef struct mpdata_ctx mpdata_ctx;
+
+/* ---- library / context ------------------------------------------------ */
+
+const char *mpdata_version(void);
+const char *mpdata_last_error(void);
+
+/* Creates the per-domain context (scratch for the antidiffusive passes) on
+ * the current HIP device.  Replaces the module-global scratch of
+ * common/scratch_datafields_modular.f90:282-334 and the single-process
+ * set-up of geomset/init_subdomains (common/mpi_parallel.f90:88-501,
+ * 606-682).  Requires np,mp,lp >= 3, ih >= 1, ibc* in {0,1},
+ * real_bytes in {4,8}. */
+int mpdata_ctx_create(int np, int mp, int lp, int ih, int ibcx, int ibcy, int ibcz,
+                      int real_bytes, mpdata_ctx **out);
+int mpdata_ctx_destroy(mpdata_ctx *ctx);
+/* Selects the execution path of mpdata_step_* / mpdata_c_timeloop_*. */
+int mpdata_ctx_set_path(mpdata_ctx *ctx, int path);
+/* Bytes of device scratch currently held by the context. */
+long long mpdata_ctx_scratch_bytes(const mpdata_ctx *ctx);
+
+/* ---- stage-level drop-ins ------------------------------------------------
+ * Same arguments, in the same order, as the reference subroutines, with the
+ * context first and the stream last (the serialization flags are dropped).
+ * np,mp,lp,ih and ibc* must match the context; ipoles0 must be 0 (the
+ * Cartesian dwarf; the reference's pole code paths are commented out).  */
+
+/* upwind3d_gpubc: advection/src_algorithms/upwind3d-gpubc.f90:20-28.
+ * Writes the x/y halo of x_in from bcx/bcy on open walls (putbcinxh) and
+ * the interior of xant.  xforc_impl/xforc_expl and dt are accepted and
+ * unused, as in the reference (:93-100 commented out); they may be NULL. */
+int mpdata_upwind3d_gpubc_f64(mpdata_ctx *ctx, const double *u1, const double *u2,
+                              const double *u3, double *x_in, double *xant,
+                              const double *xforc_impl, const double *xforc_expl,
+                              const double *rhr, const double *h, const double *bcx,
+                              const double *bcy, int ipoles0, int ibcx0, int ibcy0, int ibcz0,
+                              int np, int mp, int lp, int ih, double dt, void *stream);
+int mpdata_upwind3d_gpubc_f32(mpdata_ctx *ctx, const float *u1, const float *u2,
+                              const float *u3, float *x_in, float *xant,
+                              const float *xforc_impl, const float *xforc_expl,
+                              const float *rhr, const float *h, const float *bcx,
+                              const float *bcy, int ipoles0, int ibcx0, int ibcy0, int ibcz0,
+                              int np, int mp, int lp, int ih, float dt, void *stream);
+
+/* antidiff3d_gauge_gpubc: advection/src_algorithms/antidiff3d-gauge-gpubc.f90:14-16.
+ * Corrective pass with the non-oscillatory limiter; updates x_in in place,
+ * overwrites the x/y halos of xant (mirror) and x_in (edge copy) as the
+ * reference does.  lupdatemulti and iflip are accepted and have no effect
+ * in a single process, as in the reference. */
+int mpdata_antidiff3d_gauge_gpubc_f64(mpdata_ctx *ctx, int lupdatemulti, const double *u1,
+                                      const double *u2, const double *u3, double *xant,
+                                      double *x_in, const double *rhr, const double *h,
+                                      int iflip, int ipoles0, int ibcx0, int ibcy0, int ibcz0,
+                                      int np, int mp, int lp, int ih, void *stream);
+int mpdata_antidiff3d_gauge_gpubc_f32(mpdata_ctx *ctx, int lupdatemulti, const float *u1,
+                                      const float *u2, const float *u3, float *xant,
+                                      float *x_in, const float *rhr, const float *h, int iflip,
+                                      int ipoles0, int ibcx0, int ibcy0, int ibcz0, int np,
+                                      int mp, int lp, int ih, void *stream);
+
+/* antidiff3d_standard_gpubc: advection/src_algorithms/antidiff3d-standard-gpubc.f90:14-16 */
+int mpdata_antidiff3d_standard_gpubc_f64(mpdata_ctx *ctx, int lupdatemulti, const double *u1,
+                                         const double *u2, const double *u3, double *xant,
+                                         double *x_in, const double *rhr, const double *h,
+                                         int iflip, int ipoles0, int ibcx0, int ibcy0,
+                                         int ibcz0, int np, int mp, int lp, int ih,
+                                         void *stream);
+int mpdata_antidiff3d_standard_gpubc_f32(mpdata_ctx *ctx, int lupdatemulti, const float *u1,
+                                         const float *u2, const float *u3, float *xant,
+                                         float *x_in, const float *rhr, const float *h,
+                                         int iflip, int ipoles0, int ibcx0, int ibcy0,
+                                         int ibcz0, int np, int mp, int lp, int ih,
+                                         void *stream);
+
+/* ---- driver-level drop-in -------------------------------------------------
+ * advec_dwarf: advection/src_drivers/advec.f90:116-136.  legacyoptmode 4
+ * (two-step GPUBC; t_adv 1 gauge, 2 standard) or 6 (upwind only, then
+ * x <- xant).  lvertsplit must be 0 (the split path is commented out in the
+ * reference, :213-217).  x_lr/x_bt are the inflow values bcx/bcy.  The
+ * optional trailing arrays of the reference (wadv2, rhr2, ...) are unused
+ * there and are not part of this signature. */
+int mpdata_advec_dwarf_f64(mpdata_ctx *ctx, int legacyoptmode, int t_adv, int lupdatemulti,
+                           int lvertsplit, int np, int mp, int lp, int ih, int ipoles,
+                           int ibcx, int ibcy, int ibcz, double *x, const double *x_lr,
+                           const double *x_bt, const double *uadv, const double *vadv,
+                           const double *wadv, const double *rhr, const double *rhoadv,
+                           void *stream);
+int mpdata_advec_dwarf_f32(mpdata_ctx *ctx, int legacyoptmode, int t_adv, int lupdatemulti,
+                           int lvertsplit, int np, int mp, int lp, int ih, int ipoles,
+                           int ibcx, int ibcy, int ibcz, float *x, const float *x_lr,
+                           const float *x_bt, const float *uadv, const float *vadv,
+                           const float *wadv, const float *rhr, const float *rhoadv,
+                           void *stream);
+
+/* ---- generalised step and time loops ---------------------------------------
+ * One MPDATA time step: upwind followed by iters-1 corrective passes.
+ * variant MPDATA_UPWIND ignores iters/nonos.  iters=2, nonos=1 is the
+ * reference; iters>2 uses the limited velocity of the previous pass as the
+ * advective velocity of the next and keeps x at time level n as limiter
+ * bound; nonos=0 drops the limiter.  On return the interior of x holds the
+ * new field; the halo of x is scratch. */
+int mpdata_step_f64(mpdata_ctx *ctx, int variant, int iters, int nonos, double *x,
+                    const double *u1, const double *u2, const double *u3, const double *rhr,
+                    const double *h, const double *bcx, const double *bcy, void *stream);
+int mpdata_step_f32(mpdata_ctx *ctx, int variant, int iters, int nonos, float *x,
+                    const float *u1, const float *u2, const float *u3, const float *rhr,
+                    const float *h, const float *bcx, const float *bcy, void *stream);
+
+/* nt steps of mpdata_step_* (the time loop of
+ * advection_dwarf_test_modular.f90:185-223 without the norms). */
+int mpdata_c_timeloop_f64(mpdata_ctx *ctx, int variant, int iters, int nonos, int nt, double *x,
+                          const double *u1, const double *u2, const double *u3,
+                          const double *rhr, const double *h, const double *bcx,
+                          const double *bcy, void *stream);
+int mpdata_c_timeloop_f32(mpdata_ctx *ctx, int variant, int iters, int nonos, int nt, float *x,
+                          const float *u1, const float *u2, const float *u3, const float *rhr,
+                          const float *h, const float *bcx, const float *bcy, void *stream);
+
+/* The reference's own C-ABI declaration (advection_dwarf_test_modular.f90:
+ * 436-446, bind(c,name="upwind_c_timeloop"), never implemented there):
+ * nt donor-cell upwind steps (each x <- upwind(x)) with zero open-boundary
+ * inflow, FP64.  Accepts host or device pointers (the reference passes
+ * C_LOC of host arrays); host data is staged through the GPU.  Synchronous. */
+int upwind_c_timeloop(void *xtracer, void *uadv, void *vadv, void *wadv, void *rhr, void *h,
+                      int np, int mp, int lp, int ih, int nt);
+
+/* ---- velocity predictor: advection/src_algorithms/module_velprd.f90 -------
+ * ox, oy, oz (and u, v, w) hold three consecutive scalar fields: time level
+ * 0 (n), 1 (n+1/2 output) and 2 (n-1), i.e. the reference's (...,0:2). */
+
+/* velprd_traj0 (:15-89): level 1 = extrapolation to n+1/2 in Courant units,
+ * then the C-grid velocities u1,u2,u3 via velprdA_to_C. */
+int mpdata_velprd_traj0_f64(mpdata_ctx *ctx, double *ox, double *oy, double *oz, double *u1,
+                            double *u2, double *u3, const double *h, double dxi, double dyi,
+                            double dzi, double dt, double dt_ratio, void *stream);
+int mpdata_velprd_traj0_f32(mpdata_ctx *ctx, float *ox, float *oy, float *oz, float *u1,
+                            float *u2, float *u3, const float *h, float dxi, float dyi,
+                            float dzi, float dt, float dt_ratio, void *stream);
+/* velprdA_to_C (:166-364) on single-level A-grid fields (their open-wall
+ * halos are overwritten with the one-sided extrapolation). */
+int mpdata_velprdA_to_C_f64(mpdata_ctx *ctx, double *ox, double *oy, double *oz, double *u1,
+                            double *u2, double *u3, void *stream);
+int mpdata_velprdA_to_C_f32(mpdata_ctx *ctx, float *ox, float *oy, float *oz, float *u1,
+                            float *u2, float *u3, void *stream);
+/* time-level rotation: advection/src_algorithms/velprd_driver.f90:254-270 */
+int mpdata_velprd_rotate_f64(mpdata_ctx *ctx, double *u, double *v, double *w, double *ox,
+                             double *oy, double *oz, const double *h, double dt_ratio,
+                             void *stream);
+int mpdata_velprd_rotate_f32(mpdata_ctx *ctx, float *u, float *v, float *w, float *ox, float *oy,
+                             float *oz, const float *h, float dt_ratio, void *stream);
+
+/* ---- diagnostics (synchronous) --------------------------------------------- */
+/* compute_courants: common/diagutils.f90:588-672.  out[7] (host) = cr3d,
+ * crxy, crxz, cryz, crx, cry, crz. */
+int mpdata_compute_courants_f64(mpdata_ctx *ctx, const double *ox, const double *oy,
+                                const double *oz, const double *h, int ntimelevel, double dxi,
+                                double dyi, double dzi, double dt, double *out, void *stream);
+int mpdata_compute_courants_f32(mpdata_ctx *ctx, const float *ox, const float *oy,
+                                const float *oz, const float *h, int ntimelevel, float dxi,
+                                float dyi, float dzi, float dt, double *out, void *stream);
+/* Interior sums for the dwarf's norms (advection_test_setup.f90:472-492,
+ * 595-617): out[4] (host) = sum x^2, sum x0^2, sum (x0-x)^2, sum x
+ * (accumulated in FP64).  x0 may be NULL (its terms are then 0). */
+int mpdata_norm_sums_f64(mpdata_ctx *ctx, const double *x, const double *x0, double *out,
+                         void *stream);
+int mpdata_norm_sums_f32(mpdata_ctx *ctx, const float *x, const float *x0, double *out,
+                         void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPDATA_HIP_H */
