@@ -36,6 +36,7 @@ def _proto(real):
         "mpdata_antidiff3d_standard_gpubc": [_P, _I] + [_P] * 7 + [_I] * 9 + [_P],
         "mpdata_advec_dwarf": [_P] + [_I] * 12 + [_P] * 9,
         "mpdata_step": [_P, _I, _I, _I] + [_P] * 9,
+        "mpdata_step_oop": [_P, _I, _I, _I] + [_P] * 10,
         "mpdata_c_timeloop": [_P, _I, _I, _I, _I] + [_P] * 9,
         "mpdata_velprd_traj0": [_P] * 8 + [R] * 5 + [_P],
         "mpdata_velprdA_to_C": [_P] * 8,

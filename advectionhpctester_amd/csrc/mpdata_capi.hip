@@ -49,6 +49,7 @@ struct mpdata_ctx {
     // scratch, allocated on first use
     void* xant = nullptr;
     void* psi2 = nullptr;
+    void* xnext = nullptr;      // output buffer of the out-of-place fused step
     void* v[3] = {nullptr, nullptr, nullptr};
     void* cp = nullptr;
     void* cn = nullptr;
@@ -249,28 +250,104 @@ int step_stages(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T*
     return MPDATA_OK;
 }
 
-template <typename T>
-int step_impl(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T* u1, const T* u2,
-              const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
-              hipStream_t s) {
+int check_step_opts(int variant, int iters, int nonos) {
     if (variant < MPDATA_UPWIND || variant > MPDATA_STANDARD)
         return fail(MPDATA_EINVAL, "variant %d", variant);
     if (variant != MPDATA_UPWIND && (iters < 2 || iters > 8))
         return fail(MPDATA_EINVAL, "iters %d outside 2..8", iters);
     if (nonos != 0 && nonos != 1) return fail(MPDATA_EINVAL, "nonos %d", nonos);
-    if (any_null(x, u1, u2, u3, rhr, h)) return fail(MPDATA_EINVAL, "null field pointer");
+    return MPDATA_OK;
+}
+
+// Whether this call runs the fused kernel (after the path checks).
+template <typename T>
+int use_fused(mpdata_ctx* c, int variant, int iters, int nonos, bool* fused) {
     const bool fusable = Fused<T>::supports(c->g, variant, iters, nonos);
     if (c->path == MPDATA_PATH_FUSED && !fusable)
         return fail(MPDATA_EUNSUPPORTED,
                     "fused path supports variant %d iters %d nonos %d only for %s", variant,
                     iters, nonos, Fused<T>::scope());
-    if (c->path != MPDATA_PATH_STAGES && fusable) {
-        if ((!c->g.px && !bcx) || (!c->g.py && !bcy))
-            return fail(MPDATA_EINVAL, "open boundaries need bcx/bcy");
-        HIPCHK(Fused<T>::step(c->g, x, u1, u2, u3, rhr, h, bcx, bcy, s));
+    *fused = c->path != MPDATA_PATH_STAGES && fusable;
+    return MPDATA_OK;
+}
+
+template <typename T>
+int check_step_args(mpdata_ctx* c, const T* x, const T* u1, const T* u2, const T* u3,
+                    const T* rhr, const T* h, const T* bcx, const T* bcy) {
+    if (any_null(x, u1, u2, u3, rhr, h)) return fail(MPDATA_EINVAL, "null field pointer");
+    if ((!c->g.px && !bcx) || (!c->g.py && !bcy))
+        return fail(MPDATA_EINVAL, "open boundaries need bcx/bcy");
+    return MPDATA_OK;
+}
+
+// In-place step: x <- MPDATA(x).  The fused kernel works out of place into
+// the context's buffer, which is then copied back.
+template <typename T>
+int step_impl(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T* u1, const T* u2,
+              const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+              hipStream_t s) {
+    int rc;
+    bool fused = false;
+    if ((rc = check_step_opts(variant, iters, nonos))) return rc;
+    if ((rc = check_step_args<T>(c, x, u1, u2, u3, rhr, h, bcx, bcy))) return rc;
+    if ((rc = use_fused<T>(c, variant, iters, nonos, &fused))) return rc;
+    if (!fused)
+        return step_stages<T>(c, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy, s);
+    if ((rc = ensure(c, &c->xnext, SCAL))) return rc;
+    T* xn = (T*)c->xnext;
+    HIPCHK(Fused<T>::step(c->g, x, xn, u1, u2, u3, rhr, h, bcx, bcy, s));
+    HIPCHK(hipMemcpyAsync(x, xn, (size_t)len_of<SCAL>(c->g) * sizeof(T), hipMemcpyDeviceToDevice, s));
+    return MPDATA_OK;
+}
+
+// Out-of-place step: x_out <- MPDATA(x_in); x_in is not modified.
+template <typename T>
+int step_oop_impl(mpdata_ctx* c, int variant, int iters, int nonos, const T* x_in, T* x_out,
+                  const T* u1, const T* u2, const T* u3, const T* rhr, const T* h,
+                  const T* bcx, const T* bcy, hipStream_t s) {
+    int rc;
+    bool fused = false;
+    if ((rc = check_step_opts(variant, iters, nonos))) return rc;
+    if ((rc = check_step_args<T>(c, x_in, u1, u2, u3, rhr, h, bcx, bcy))) return rc;
+    if (!x_out) return fail(MPDATA_EINVAL, "null x_out");
+    const size_t bytes = (size_t)len_of<SCAL>(c->g) * sizeof(T);
+    const char *a = (const char*)x_in, *b = (const char*)x_out;
+    if (a < b + bytes && b < a + bytes) return fail(MPDATA_EINVAL, "x_in and x_out overlap");
+    if ((rc = use_fused<T>(c, variant, iters, nonos, &fused))) return rc;
+    if (fused) {
+        HIPCHK(Fused<T>::step(c->g, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, s));
         return MPDATA_OK;
     }
-    return step_stages<T>(c, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy, s);
+    HIPCHK(hipMemcpyAsync(x_out, x_in, bytes, hipMemcpyDeviceToDevice, s));
+    return step_stages<T>(c, variant, iters, nonos, x_out, u1, u2, u3, rhr, h, bcx, bcy, s);
+}
+
+// nt steps.  With the fused kernel the steps alternate between x and the
+// context's buffer; an odd count ends with one copy back into x.
+template <typename T>
+int timeloop_impl(mpdata_ctx* c, int variant, int iters, int nonos, int nt, T* x, const T* u1,
+                  const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
+                  const T* bcy, hipStream_t s) {
+    int rc;
+    bool fused = false;
+    if (nt < 0) return fail(MPDATA_EINVAL, "nt %d", nt);
+    if ((rc = check_step_opts(variant, iters, nonos))) return rc;
+    if ((rc = check_step_args<T>(c, x, u1, u2, u3, rhr, h, bcx, bcy))) return rc;
+    if ((rc = use_fused<T>(c, variant, iters, nonos, &fused))) return rc;
+    if (!fused) {
+        for (int it = 0; it < nt; it++)
+            if ((rc = step_stages<T>(c, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy, s)))
+                return rc;
+        return MPDATA_OK;
+    }
+    if ((rc = ensure(c, &c->xnext, SCAL))) return rc;
+    T* buf[2] = {x, (T*)c->xnext};
+    for (int it = 0; it < nt; it++)
+        HIPCHK(Fused<T>::step(c->g, buf[it & 1], buf[(it + 1) & 1], u1, u2, u3, rhr, h, bcx, bcy, s));
+    if (nt & 1)
+        HIPCHK(hipMemcpyAsync(x, buf[1], (size_t)len_of<SCAL>(c->g) * sizeof(T),
+                              hipMemcpyDeviceToDevice, s));
+    return MPDATA_OK;
 }
 
 template <typename T>
@@ -439,12 +516,16 @@ long long mpdata_ctx_scratch_bytes(const mpdata_ctx* ctx) { return ctx ? ctx->he
                                const T* bcx, const T* bcy, void* stream) {                       \
         int rc;                                                                                  \
         if ((rc = check_ctx(ctx, RB))) return rc;                                                \
-        if (nt < 0) return fail(MPDATA_EINVAL, "nt %d", nt);                                     \
-        for (int it = 0; it < nt; it++)                                                          \
-            if ((rc = step_impl<T>(ctx, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy,  \
-                                   (hipStream_t)stream)))                                        \
-                return rc;                                                                       \
-        return MPDATA_OK;                                                                        \
+        return timeloop_impl<T>(ctx, variant, iters, nonos, nt, x, u1, u2, u3, rhr, h, bcx, bcy, \
+                                (hipStream_t)stream);                                            \
+    }                                                                                            \
+    int mpdata_step_oop##SFX(mpdata_ctx* ctx, int variant, int iters, int nonos, const T* x_in,  \
+                             T* x_out, const T* u1, const T* u2, const T* u3, const T* rhr,      \
+                             const T* h, const T* bcx, const T* bcy, void* stream) {             \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        return step_oop_impl<T>(ctx, variant, iters, nonos, x_in, x_out, u1, u2, u3, rhr, h,     \
+                                bcx, bcy, (hipStream_t)stream);                                  \
     }                                                                                            \
     int mpdata_velprd_traj0##SFX(mpdata_ctx* ctx, T* ox, T* oy, T* oz, T* u1, T* u2, T* u3,      \
                                  const T* h, T dxi, T dyi, T dzi, T dt, T dt_ratio,              \

@@ -10,10 +10,12 @@ struct Fused {
     static bool supports(const Geo& g, int variant, int iters, int nonos);
     // Human-readable scope of the fused kernel, for error messages.
     static const char* scope();
-    // One full step x <- MPDATA(x); reads x (and bcx/bcy on open walls),
-    // writes the interior of x through a ping-pong buffer held per device.
-    static hipError_t step(const Geo& g, T* x, const T* u1, const T* u2, const T* u3,
-                           const T* rhr, const T* h, const T* bcx, const T* bcy, hipStream_t s);
+    // One full step out of place: xout <- MPDATA(x).  xout ends up as the
+    // reference would leave x (new interior, edge-copied x/y halo, other
+    // halo cells from x).  x and xout must not overlap.
+    static hipError_t step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                           const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+                           hipStream_t s);
 };
 
 }  // namespace mpdata

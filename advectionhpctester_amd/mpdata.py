@@ -178,6 +178,15 @@ class Domain:
         check(self._fn("mpdata_step")(self._ctx, variant, iters, nonos, self._p(x, 0, "x"), pu1,
                                       pu2, pu3, prhr, ph, pbx, pby, self._stream()))
 
+    def step_oop(self, x_in, x_out, u1, u2, u3, rhr, h, bcx=None, bcy=None, variant=GAUGE,
+                 iters=2, nonos=1):
+        """x_out <- MPDATA(x_in) without modifying x_in."""
+        pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
+        pbx, pby = self._bc(bcx, bcy)
+        check(self._fn("mpdata_step_oop")(self._ctx, variant, iters, nonos,
+                                          self._p(x_in, 0, "x_in"), self._p(x_out, 0, "x_out"),
+                                          pu1, pu2, pu3, prhr, ph, pbx, pby, self._stream()))
+
     def timeloop(self, nt, x, u1, u2, u3, rhr, h, bcx=None, bcy=None, variant=GAUGE, iters=2,
                  nonos=1):
         pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)

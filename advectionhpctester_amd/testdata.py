@@ -151,3 +151,114 @@ def random_case(np_, mp, lp, ih=1, seed=0, dtype=np.float64, cmax=0.3, positive=
     bcx = rng.uniform(0.5, 2.0, (2, lp, mp)).astype(dtype)
     bcy = rng.uniform(0.5, 2.0, (2, lp, np_)).astype(dtype)
     return dict(x=x, u1=u1, u2=u2, u3=u3, h=h, rhr=rhr, bcx=bcx, bcy=bcy)
+
+
+# ---------------------------------------------------------------------------
+# Synthetic workloads (torch, on any device)
+# ---------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def courant_3d(ua, va, wa):
+    """max over cells of |u|+|v|+|w| (A-grid, Courant units): the cr3d of
+    common/diagutils.f90:623-634."""
+    return float((ua.abs() + va.abs() + wa.abs()).max())
+
+
+def solid_body_a_grid(np_, mp, lp, axis=(1.0, 1.0, 1.0), cmax=0.8, ih=1, dtype=None,
+                      device="cpu"):
+    """A-grid solid-body rotation about an axis through the domain centre,
+    scaled so that max(|u|+|v|+|w|) = cmax.  Returns scalar-shaped tensors
+    (halo cells zero)."""
+    torch = _torch()
+    dtype = dtype or torch.float64
+    ax = torch.tensor(axis, dtype=torch.float64)
+    ax = ax / ax.norm()
+    # cell centres relative to the domain centre, in cell units
+    xi = torch.arange(1, np_ + 1, dtype=torch.float64, device=device) - (np_ + 1) / 2
+    yj = torch.arange(1, mp + 1, dtype=torch.float64, device=device) - (mp + 1) / 2
+    zk = torch.arange(1, lp + 1, dtype=torch.float64, device=device) - (lp + 1) / 2
+    Z, Y, X = torch.meshgrid(zk, yj, xi, indexing="ij")
+    ox, oy, oz = (float(a) for a in ax)
+    u = oy * Z - oz * Y
+    v = oz * X - ox * Z
+    w = ox * Y - oy * X
+    s = cmax / courant_3d(u, v, w)
+    out = []
+    for a in (u, v, w):
+        f = torch.zeros(shape(np_, mp, lp, ih), dtype=dtype, device=device)
+        f[ih:ih + lp, ih:ih + mp, ih:ih + np_] = (a * s).to(dtype)
+        out.append(f)
+    return out
+
+
+def a_to_c_torch(ua, va, wa, ih=1):
+    """A-grid -> C-grid on open walls (module_velprd.f90:166-364), torch."""
+    torch = _torch()
+    lp, mp, np_ = (s - 2 * ih for s in ua.shape)
+    I, J, K = slice(ih, ih + np_), slice(ih, ih + mp), slice(ih, ih + lp)
+    ua, va, wa = ua.clone(), va.clone(), wa.clone()
+    ua[K, J, ih - 1] = 2.0 * ua[K, J, ih] - ua[K, J, ih + 1]
+    ua[K, J, ih + np_] = 2.0 * ua[K, J, ih + np_ - 1] - ua[K, J, ih + np_ - 2]
+    va[K, ih - 1, I] = 2.0 * va[K, ih, I] - va[K, ih + 1, I]
+    va[K, ih + mp, I] = 2.0 * va[K, ih + mp - 1, I] - va[K, ih + mp - 2, I]
+    wa[ih - 1, J, I] = 2.0 * wa[ih, J, I] - wa[ih + 1, J, I]
+    wa[ih + lp, J, I] = 2.0 * wa[ih + lp - 1, J, I] - wa[ih + lp - 2, J, I]
+    kw = dict(dtype=ua.dtype, device=ua.device)
+    u1 = torch.zeros(shape(np_, mp, lp, ih, 1), **kw)
+    u2 = torch.zeros(shape(np_, mp, lp, ih, 2), **kw)
+    u3 = torch.zeros(shape(np_, mp, lp, ih, 3), **kw)
+    u1[K, J, ih:ih + np_ + 1] = 0.5 * (ua[K, J, ih:ih + np_ + 1] + ua[K, J, ih - 1:ih + np_])
+    u2[K, ih:ih + mp + 1, I] = 0.5 * (va[K, ih:ih + mp + 1, I] + va[K, ih - 1:ih + mp, I])
+    u3[ih:ih + lp + 1, J, I] = 0.5 * (wa[ih:ih + lp + 1, J, I] + wa[ih - 1:ih + lp, J, I])
+    return u1, u2, u3
+
+
+def fourier_field(np_, mp, lp, kmax=8, nmodes=24, seed=3, amplitude=1.0, ih=1, dtype=None,
+                  device="cpu"):
+    """Smooth random field: sum of nmodes separable Fourier modes with integer
+    wavenumbers 0..kmax per direction (over the domain length), random phases
+    and amplitudes, normalised to max |f - mean| = amplitude and offset to
+    stay positive (minimum 0.1)."""
+    torch = _torch()
+    dtype = dtype or torch.float64
+    g = torch.Generator().manual_seed(seed)
+    kk = torch.randint(0, kmax + 1, (nmodes, 3), generator=g)
+    ph = torch.rand((nmodes, 3), generator=g, dtype=torch.float64) * 2 * math.pi
+    am = torch.rand((nmodes,), generator=g, dtype=torch.float64) * 2 - 1
+    xi = (torch.arange(np_, dtype=torch.float64, device=device) + 0.5) / np_
+    yj = (torch.arange(mp, dtype=torch.float64, device=device) + 0.5) / mp
+    zk = (torch.arange(lp, dtype=torch.float64, device=device) + 0.5) / lp
+    f = torch.zeros((lp, mp, np_), dtype=torch.float64, device=device)
+    for m in range(nmodes):
+        cx = torch.cos(2 * math.pi * int(kk[m, 0]) * xi + float(ph[m, 0]))
+        cy = torch.cos(2 * math.pi * int(kk[m, 1]) * yj + float(ph[m, 1]))
+        cz = torch.cos(2 * math.pi * int(kk[m, 2]) * zk + float(ph[m, 2]))
+        f += float(am[m]) * cz[:, None, None] * cy[None, :, None] * cx[None, None, :]
+    f -= f.mean()
+    f *= amplitude / f.abs().max()
+    f += 0.1 - f.min()
+    out = torch.zeros(shape(np_, mp, lp, ih), dtype=dtype, device=device)
+    out[ih:ih + lp, ih:ih + mp, ih:ih + np_] = f.to(dtype)
+    return out
+
+
+def tilted_rotation_case(np_=512, mp=512, lp=256, cmax=0.8, seed=3, ih=1, dtype=None,
+                         device="cpu"):
+    """The single large instance: Fourier-mode field (k <= 8, amplitude 1,
+    positive) advected by solid-body rotation about the (1,1,1) axis with max
+    Courant cmax, h = rhr = 1, zero inflow on the open walls."""
+    torch = _torch()
+    dtype = dtype or torch.float64
+    x = fourier_field(np_, mp, lp, seed=seed, ih=ih, dtype=dtype, device=device)
+    ua, va, wa = solid_body_a_grid(np_, mp, lp, cmax=cmax, ih=ih, dtype=torch.float64,
+                                   device=device)
+    u1, u2, u3 = (a.to(dtype) for a in a_to_c_torch(ua, va, wa, ih))
+    del ua, va, wa
+    h = torch.ones(shape(np_, mp, lp, ih), dtype=dtype, device=device)
+    rhr = torch.ones_like(h)
+    bcx = torch.zeros((2, lp, mp), dtype=dtype, device=device)
+    bcy = torch.zeros((2, lp, np_), dtype=dtype, device=device)
+    return dict(x=x, u1=u1, u2=u2, u3=u3, h=h, rhr=rhr, bcx=bcx, bcy=bcy)

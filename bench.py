@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Throughput of the MPDATA step on one MI355X.
+
+Workload (default): the single large instance -- 512 x 512 x 256 cells, FP64,
+infinite-gauge non-oscillatory MPDATA with two iterations (the reference's
+DWARF3 scheme, advec.f90 legacyoptmode 4 / t_adv 1), open boundaries,
+solid-body rotation about the (1,1,1) axis with maximum Courant number 0.8,
+and a positive random Fourier-mode field (k <= 8).  Inputs are generated on
+the GPU and resident in HBM before the timed region.
+
+A step is one call of mpdata_step_oop (x_out <- MPDATA(x_in)); the timed loop
+alternates two field buffers, which is how the library's own time loop runs.
+Prints one JSON line:
+  value      cell-updates/s (interior cells x steps / seconds)
+  roofline   achieved = algorithmic bytes per step (56 B per cell: read x,
+             u1, u2, u3, h, rhr, write x) / average step time, against the
+             MI355X HBM peak of 8 TB/s; traffic = measured HBM bytes per step
+             from the committed rocprofv3 counter summary, when present
+  cpu_baseline  the CPU oracle (plain C, one thread) on one step of the same
+             workload, timed on this host
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MPDATA cell-updates/s, % HBM roofline; adjoint fwd+bwd time vs plain fwd"
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, nargs=3, default=(512, 512, 256), metavar=("NP", "MP", "LP"))
+    ap.add_argument("--dtype", choices=("f64", "f32"), default="f64")
+    ap.add_argument("--path", choices=("auto", "fused", "stages"), default="auto")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="steps of the CPU baseline sample (0 disables it)")
+    ap.add_argument("--no-check", action="store_true", help="skip the fused-vs-stages check")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the step shards by 3-D domain decomposition, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from advectionhpctester_amd import mpdata, testdata
+
+    tdt = torch.float64 if a.dtype == "f64" else torch.float32
+    np_, mp_, lp_ = a.size
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    case = testdata.tilted_rotation_case(np_, mp_, lp_, cmax=0.8, dtype=tdt, device=dev)
+    dom = mpdata.Domain(np_, mp_, lp_, dtype=tdt, device=dev)
+    dom.set_path({"auto": mpdata.PATH_AUTO, "fused": mpdata.PATH_FUSED,
+                  "stages": mpdata.PATH_STAGES}[a.path])
+    bufs = [case["x"], torch.empty_like(case["x"])]
+    bufs[1].copy_(bufs[0])
+    args = (case["u1"], case["u2"], case["u3"], case["rhr"], case["h"], case["bcx"], case["bcy"])
+
+    def run(n, start):
+        for s in range(start, start + n):
+            dom.step_oop(bufs[s & 1], bufs[(s + 1) & 1], *args)
+        return start + n
+
+    s = run(a.warmup, 0)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    s = run(a.steps, s)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    cells = np_ * mp_ * lp_
+    ms_step = gpu_ms / max(a.steps, 1)
+    value = cells * a.steps / (gpu_ms / 1e3)
+    rb = 8 if a.dtype == "f64" else 4
+    alg_bytes = 7 * rb * cells
+    achieved = alg_bytes / (ms_step / 1e3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        key = f"{np_}x{mp_}x{lp_}_{a.dtype}"
+        if key in t:
+            traffic = t[key]["bytes_per_step"]
+
+    # correctness of what was timed: fused and per-stage paths agree bit for bit
+    checked = None
+    if not a.no_check:
+        x0 = bufs[s & 1]
+        xf = torch.empty_like(x0)
+        dom.step_oop(x0, xf, *args)
+        xs = x0.clone()
+        dom.set_path(mpdata.PATH_STAGES)
+        dom.step(xs, *args)
+        torch.cuda.synchronize()
+        checked = bool(torch.equal(xf, xs))
+        del xf, xs
+
+    cpu = None
+    if a.cpu_steps > 0:
+        cpu = cpu_baseline(case, (np_, mp_, lp_, 1), a.dtype, a.cpu_steps)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.dtype,
+        "data": "synthetic",
+        "config": {
+            "workload": f"single large instance {np_}x{mp_}x{lp_}: gauge MPDATA, 2 iterations, "
+                        "non-oscillatory, open walls, solid-body rotation about (1,1,1) at "
+                        "Courant 0.8, Fourier-mode field k<=8",
+            "grid": [np_, mp_, lp_], "variant": "gauge", "iters": 2, "nonos": 1,
+            "boundaries": "open", "path": a.path, "parallelism": "single GPU",
+        },
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_step": alg_bytes},
+        "cpu_baseline": cpu,
+        "checked_fused_equals_stages": checked,
+        "wall_s_timed": wall,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(case, dims, dtype, steps):
+    """The oracle (plain C, one thread) on `steps` steps of the same workload."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+
+    npdt = np.float64 if dtype == "f64" else np.float32
+    host = {k: v.cpu().numpy() for k, v in case.items()}
+    orc = Oracle(npdt)
+    x = host["x"].copy()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        orc.step(1, 2, 1, x, host["u1"], host["u2"], host["u3"], host["rhr"], host["h"],
+                 host["bcx"], host["bcy"], (0, 0, 0), dims)
+    dt = time.perf_counter() - t0
+    cells = dims[0] * dims[1] * dims[2]
+    return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} step(s) of the same {dims[0]}x{dims[1]}x{dims[2]} {dtype} workload "
+                      f"with oracle/liboracle_mpdata.so (plain C restatement, -O2, one thread), "
+                      f"{dt:.1f} s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -165,8 +165,24 @@ int mpdata_step_f32(mpdata_ctx *ctx, int variant, int iters, int nonos, float *x
                     const float *u1, const float *u2, const float *u3, const float *rhr,
                     const float *h, const float *bcx, const float *bcy, void *stream);
 
+/* Out-of-place step: x_out <- MPDATA(x_in); x_in is left unchanged and
+ * x_out receives the whole array as the in-place step would leave x (new
+ * interior, halo as the reference leaves it).  x_in and x_out must not
+ * overlap.  The fused kernel writes x_out directly, so a caller that
+ * alternates two buffers moves each field through HBM once per step. */
+int mpdata_step_oop_f64(mpdata_ctx *ctx, int variant, int iters, int nonos, const double *x_in,
+                        double *x_out, const double *u1, const double *u2, const double *u3,
+                        const double *rhr, const double *h, const double *bcx,
+                        const double *bcy, void *stream);
+int mpdata_step_oop_f32(mpdata_ctx *ctx, int variant, int iters, int nonos, const float *x_in,
+                        float *x_out, const float *u1, const float *u2, const float *u3,
+                        const float *rhr, const float *h, const float *bcx, const float *bcy,
+                        void *stream);
+
 /* nt steps of mpdata_step_* (the time loop of
- * advection_dwarf_test_modular.f90:185-223 without the norms). */
+ * advection_dwarf_test_modular.f90:185-223 without the norms).  On the
+ * fused path the steps alternate between x and a context buffer, with one
+ * final copy into x when nt is odd. */
 int mpdata_c_timeloop_f64(mpdata_ctx *ctx, int variant, int iters, int nonos, int nt, double *x,
                           const double *u1, const double *u2, const double *u3,
                           const double *rhr, const double *h, const double *bcx,
