@@ -145,8 +145,8 @@ bool any_null(P... p) {
     return false;
 }
 
-// One corrective pass on the per-stage kernels (see orc_antidiff_pass of the
-// oracle for the sequence; antidiff3d-gauge-gpubc.f90:71-430,
+// One corrective pass on the per-stage kernels, in the reference's stage
+// order (antidiff3d-gauge-gpubc.f90:71-430,
 // antidiff3d-standard-gpubc.f90:59-462).
 template <typename T>
 int stage_pass(mpdata_ctx* c, int variant, int nonos, const T* u1, const T* u2, const T* u3,

@@ -91,6 +91,29 @@ template <typename T> __device__ __forceinline__ T min2(T a, T b) { return a < b
 template <typename T> __device__ __forceinline__ T max2(T a, T b) { return a > b ? a : b; }
 template <typename T> __device__ __forceinline__ T min3(T a, T b, T c) { return min2(min2(a, b), c); }
 
+// Hardware max/min (v_max_f64 / v_min_f64: one instruction instead of a
+// compare and two selects).  For finite operands they return the same value
+// as the selects above except possibly the sign of a zero result.  The fused
+// kernel uses them where a signed zero cannot reach the new field: every
+// zero that max(0,v) or min(0,v) produces is multiplied into a flux, the flux
+// sum is subtracted from a field value, and x - (+-0) = x in round-to-nearest;
+// limiter bounds and ratios are max/min/sums of finite values none of which
+// is -0 (a field value is -0 only if the input holds -0).
+namespace hw {
+template <typename T> __device__ __forceinline__ T max2(T a, T b) { return fmax(a, b); }
+template <typename T> __device__ __forceinline__ T min2(T a, T b) { return fmin(a, b); }
+template <typename T> __device__ __forceinline__ T min3(T a, T b, T c) { return fmin(fmin(a, b), c); }
+template <typename T> __device__ __forceinline__ T pp(T y) { return fmax(T(0), y); }
+template <typename T> __device__ __forceinline__ T pn(T y) { return -fmin(T(0), y); }
+template <typename T> __device__ __forceinline__ T donor(T y1, T y2, T a) {
+    return fmax(T(0), a) * y1 - (-(fmin(T(0), a) * y2));
+}
+template <typename T>
+__device__ __forceinline__ T gflux(T v, T cp_c, T cn_m, T cp_m, T cn_c) {
+    return pp(v) * min3(T(1), cp_c, cn_m) - pn(v) * min3(T(1), cp_m, cn_c);
+}
+}  // namespace hw
+
 // Gauge limited flux through one face (antidiff3d-gauge-gpubc.f90:405-422):
 // v the pseudo-velocity, cp_c/cn_c at the downstream cell, cp_m/cn_m upstream.
 template <typename T>

@@ -1,0 +1,124 @@
+"""Writes the golden fixtures of tests/golden from the reference's own compiled
+Fortran (oracle/_ref/libmpdata_ref.so, built by oracle/build_ref.sh from the
+advectionHPCtester sources).  Run in the container that holds the reference:
+
+    python tests/golden/make_golden.py
+
+Each fixture is data only: seeded inputs and the reference's outputs.
+"""
+import hashlib
+import io
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, ROOT)
+
+from oracle import Reference, shape  # noqa: E402
+
+from advectionhpctester_amd import testdata  # noqa: E402
+
+
+def interior_hash(x, ih=1):
+    a = np.ascontiguousarray(x[ih:-ih, ih:-ih, ih:-ih])
+    return hashlib.sha256(a.tobytes()).hexdigest()
+
+
+def stage_case(ref, n, lp, seed):
+    dims = (n, n, lp, 1)
+    c = testdata.random_case(*dims, seed=seed)
+    ref.setup(n, n, lp, 1)
+    out = {f"in_{k}": v for k, v in c.items()}
+    x, xant = c["x"].copy(), np.zeros(shape(*dims))
+    ref.upwind(c["u1"], c["u2"], c["u3"], x, xant, c["rhr"], c["h"], c["bcx"], c["bcy"])
+    out["up_x"], out["up_xant"] = x.copy(), xant.copy()
+    for variant, name in ((1, "gauge"), (2, "standard")):
+        xv, av = x.copy(), xant.copy()
+        s = {k: np.zeros(shape(*dims, kind)) for k, kind in
+             (("v1", 1), ("v2", 2), ("v3", 3), ("cp", 0), ("cn", 0), ("f1", 1), ("f2", 2),
+              ("f3", 3), ("f1ns", 1), ("f2ns", 2), ("f3ns", 3))}
+        ref.antidiff(variant, c["u1"], c["u2"], c["u3"], av, xv, c["rhr"], c["h"], s)
+        out[f"{name}_x"], out[f"{name}_xant"] = xv, av
+        keys = ("v1", "v2", "v3", "cp", "cn") + (("f1", "f2", "f3", "f1ns", "f2ns", "f3ns")
+                                                 if variant == 2 else ())
+        for k in keys:
+            out[f"{name}_{k}"] = s[k]
+    return out
+
+
+def velprd_case(ref, n, lp, seed):
+    rng = np.random.default_rng(seed)
+    dims = (n, n, lp, 1)
+    ref.setup(n, n, lp, 1)
+    out = {}
+    S = shape(*dims)
+    ox, oy, oz = (rng.uniform(-1, 1, (3,) + S) for _ in range(3))
+    h = rng.uniform(0.8, 1.2, S)
+    out["in_ox"], out["in_oy"], out["in_oz"], out["in_h"] = ox.copy(), oy.copy(), oz.copy(), h
+    u1, u2, u3 = (np.zeros(shape(*dims, k)) for k in (1, 2, 3))
+    ref.velprd_traj0(ox, oy, oz, u1, u2, u3, h, 0.7, 0.9, 1.1, 0.05, 0.8)
+    out["traj0_ox"], out["traj0_oy"], out["traj0_oz"] = ox, oy, oz
+    out["traj0_u1"], out["traj0_u2"], out["traj0_u3"] = u1, u2, u3
+    a = [rng.uniform(-1, 1, S) for _ in range(3)]
+    out["a2c_in"] = np.stack(a)
+    for ver in (1, 2):
+        aa = [v.copy() for v in a]
+        c1, c2, c3 = (np.zeros(shape(*dims, k)) for k in (1, 2, 3))
+        ref.velprdA_to_C(ver, *aa, c1, c2, c3)
+        out[f"a2c{ver}_u1"], out[f"a2c{ver}_u2"], out[f"a2c{ver}_u3"] = c1, c2, c3
+        out[f"a2c{ver}_a"] = np.stack(aa)
+    cr = ref.compute_courants(out["in_ox"], out["in_oy"], out["in_oz"], h, 1, 0.7, 0.9, 1.1, 0.05)
+    cr2 = ref.compute_courants(out["traj0_ox"], out["traj0_oy"], out["traj0_oz"], h, 2, 0.7, 0.9,
+                               1.1, 0.05)
+    out["courants_1"], out["courants_2"] = cr, cr2
+    return out
+
+
+def dwarf(ref, steps_hash=(1, 50, 100)):
+    case = testdata.dwarf_setup()
+    n = case.n
+    ref.setup(n, n, n, 1, dt=case.dt, nt=case.nt)
+    u1, u2, u3 = (np.zeros(shape(n, n, n, 1, k)) for k in (1, 2, 3))
+    x = np.zeros(shape(n, n, n, 1))
+    ref.set_initial15(u1, u2, u3, x, np.ones(shape(n, n, n, 1)))
+    hashes = {}
+    for it in range(1, case.nt + 1):
+        ref.advec_dwarf(4, 1, x, case.bcx, case.bcy, u1, u2, u3, case.rhr, case.h)
+        if it in steps_hash:
+            hashes[it] = interior_hash(x)
+    buf = io.StringIO()
+    # the reference prints its norms from Fortran (fd 1); capture by file
+    import tempfile
+    with tempfile.TemporaryFile(mode="w+") as tf:
+        fd = os.dup(1)
+        os.dup2(tf.fileno(), 1)
+        try:
+            ref.norms(x, case.x0, case.nt)
+        finally:
+            os.dup2(fd, 1)
+            os.close(fd)
+        tf.seek(0)
+        buf.write(tf.read())
+    return hashes, buf.getvalue(), x
+
+
+def main():
+    ref = Reference()
+    np.savez_compressed(os.path.join(HERE, "stage_n12_l10.npz"), **stage_case(ref, 12, 10, 1))
+    np.savez_compressed(os.path.join(HERE, "stage_n9_l6.npz"), **stage_case(ref, 9, 6, 2))
+    np.savez_compressed(os.path.join(HERE, "velprd_n8_l6.npz"), **velprd_case(ref, 8, 6, 3))
+    hashes, text, x = dwarf(ref)
+    lines = [f"step {k} interior_sha256 {v}" for k, v in sorted(hashes.items())]
+    lines += ["reference norms after 556 steps:"] + [ln.strip() for ln in text.splitlines() if ln.strip()]
+    lines.append(f"final interior_sha256 {interior_hash(x)}")
+    with open(os.path.join(HERE, "dwarf59_reference.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    main()
