@@ -632,4 +632,43 @@ int upwind_c_timeloop(void* xtracer, void* uadv, void* vadv, void* wadv, void* r
     return rc;
 }
 
+int mpdata_dev_alloc(long long bytes, void** ptr) {
+    if (!ptr) return fail(MPDATA_EINVAL, "null output pointer");
+    *ptr = nullptr;
+    if (bytes <= 0) return fail(MPDATA_EINVAL, "bytes must be positive (got %lld)", bytes);
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, (size_t)bytes);
+    if (e != hipSuccess) return fail(MPDATA_ENOMEM, "hipMalloc(%lld): %s", bytes, hipGetErrorString(e));
+    e = hipMemset(p, 0, (size_t)bytes);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return hip_fail(e, "hipMemset");
+    }
+    *ptr = p;
+    return MPDATA_OK;
+}
+
+int mpdata_dev_free(void* ptr) {
+    if (!ptr) return MPDATA_OK;
+    HIPCHK(hipFree(ptr));
+    return MPDATA_OK;
+}
+
+int mpdata_copy(void* dst, const void* src, long long bytes, int kind) {
+    static const hipMemcpyKind kinds[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost,
+                                           hipMemcpyDeviceToDevice};
+    if (kind < 0 || kind > 2) return fail(MPDATA_EINVAL, "copy kind %d (0 h2d, 1 d2h, 2 d2d)", kind);
+    if (bytes < 0) return fail(MPDATA_EINVAL, "negative byte count");
+    if (bytes == 0) return MPDATA_OK;
+    if (!dst || !src) return fail(MPDATA_EINVAL, "null pointer");
+    HIPCHK(hipMemcpy(dst, src, (size_t)bytes, kinds[kind]));
+    return MPDATA_OK;
+}
+
+int mpdata_sync(void* stream) {
+    if (stream) HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    else HIPCHK(hipDeviceSynchronize());
+    return MPDATA_OK;
+}
+
 }  // extern "C"

@@ -262,3 +262,113 @@ def tilted_rotation_case(np_=512, mp=512, lp=256, cmax=0.8, seed=3, ih=1, dtype=
     bcx = torch.zeros((2, lp, mp), dtype=dtype, device=device)
     bcy = torch.zeros((2, lp, np_), dtype=dtype, device=device)
     return dict(x=x, u1=u1, u2=u2, u3=u3, h=h, rhr=rhr, bcx=bcx, bcy=bcy)
+
+
+# ---------------------------------------------------------------------------
+# Plain periodic case and time-varying deformational flow (numpy, host)
+# ---------------------------------------------------------------------------
+def periodic_rotation_case(np_=64, mp=64, lp=32, cmax=0.5, radius=8.0, amplitude=1.0,
+                           background=0.1, noise=1e-3, seed=1, ih=1, dtype=np.float64):
+    """Periodic in all directions: a Gaussian sphere exp(-r^2/radius^2) (cells)
+    of the given amplitude, centred a quarter of the domain width from the
+    rotation axis (at x = xc + np/4), on a background value, plus
+    seeded uniform noise in [-noise, noise]; solid-body rotation about the z
+    axis through the centre, w = 0.  Face velocities are the exact rotation
+    at the faces (u depends on y only, v on x only: discretely
+    divergence-free, also across the periodic seams), scaled so that the
+    A-grid max of |u|+|v| is cmax; h = rhr = 1."""
+    rng = np.random.default_rng(seed)
+    xc, yc, zc = (np_ + 1) / 2, (mp + 1) / 2, (lp + 1) / 2
+    k, j, i = np.meshgrid(np.arange(1, lp + 1), np.arange(1, mp + 1), np.arange(1, np_ + 1),
+                          indexing="ij")
+    r2 = (i - xc - np_ / 4) ** 2 + (j - yc) ** 2 + (k - zc) ** 2
+    x = np.zeros(shape(np_, mp, lp, ih))
+    x[ih:ih + lp, ih:ih + mp, ih:ih + np_] = (background + amplitude * np.exp(-r2 / radius ** 2)
+                                              + rng.uniform(-noise, noise, r2.shape))
+    omega = cmax / ((np_ - 1) / 2 + (mp - 1) / 2)
+    u1 = np.zeros(shape(np_, mp, lp, ih, 1))
+    u2 = np.zeros(shape(np_, mp, lp, ih, 2))
+    u3 = np.zeros(shape(np_, mp, lp, ih, 3))
+    jj = np.arange(1 - ih, mp + ih + 1)      # y index of every row of u1
+    ii = np.arange(1 - ih, np_ + ih + 1)     # x index of every column of u2
+    u1[:, :, :] = (-omega * (jj - yc))[None, :, None]
+    u2[:, :, :] = (omega * (ii - xc))[None, None, :]
+    h = np.ones(shape(np_, mp, lp, ih))
+    cast = lambda a: a.astype(dtype)  # noqa: E731
+    return dict(x=cast(x), u1=cast(u1), u2=cast(u2), u3=cast(u3), h=cast(h), rhr=cast(h.copy()),
+                bcx=np.zeros((2, lp, mp), dtype), bcy=np.zeros((2, lp, np_), dtype))
+
+
+def deformational_velocity(np_, mp, lp, cmax=0.7, ih=1):
+    """A-grid velocity of the 3-D deformational flow on the unit cube (cell
+    centres at (i-1/2)/np ...), at full strength, in the continuum
+        u =  2 sin^2(pi x) sin(2 pi y) sin(2 pi z)
+        v =   -sin(2 pi x) sin^2(pi y) sin(2 pi z)
+        w =   -sin(2 pi x) sin(2 pi y) sin^2(pi z),
+    the curl of the vector potential (0, -B, A) with
+    A = sin^2(pi x) sin^2(pi y) sin(2 pi z) / pi and
+    B = sin^2(pi x) sin^2(pi z) sin(2 pi y) / pi.  On the grid the curl is
+    taken with centred differences of the potential at the cell centres, so
+    the C-grid velocity that the A-to-C averaging of the velocity predictor
+    makes from it has zero discrete divergence (up to round-off) away from the
+    walls; at the walls the flow vanishes to second order.  Scaled so that the
+    max over cells of |u|+|v|+|w| is cmax (Courant units).  The
+    time-dependent flow is this field times cos(pi t / T)
+    (:func:`reversal_factor`)."""
+    def centres(n):
+        return (np.arange(1 - ih, n + ih + 1) - 0.5) / n   # cells 1-ih .. n+ih
+
+    xs, ys, zs = centres(np_), centres(mp), centres(lp)
+    s2 = lambda a: np.sin(np.pi * a) ** 2   # noqa: E731
+    sn = lambda a: np.sin(2 * np.pi * a)    # noqa: E731
+    A = (s2(xs)[None, None, :] * s2(ys)[None, :, None] * sn(zs)[:, None, None]) / np.pi
+    B = (s2(xs)[None, None, :] * sn(ys)[None, :, None] * s2(zs)[:, None, None]) / np.pi
+
+    def d(f, axis):   # centred difference in cell units, interior cells only
+        sl_p = [slice(ih, -ih)] * 3
+        sl_m = [slice(ih, -ih)] * 3
+        sl_p[axis] = slice(ih + 1, f.shape[axis] - ih + 1)
+        sl_m[axis] = slice(ih - 1, f.shape[axis] - ih - 1)
+        return 0.5 * (f[tuple(sl_p)] - f[tuple(sl_m)])
+
+    # axes of the arrays: 0 = z, 1 = y, 2 = x
+    u = d(A, 1) + d(B, 0)
+    v = -d(A, 2)
+    w = -d(B, 2)
+    s = cmax / float((np.abs(u) + np.abs(v) + np.abs(w)).max())
+    out = []
+    for a in (u, v, w):
+        f = np.zeros(shape(np_, mp, lp, ih))
+        f[ih:ih + lp, ih:ih + mp, ih:ih + np_] = a * s
+        out.append(f)
+    return out
+
+
+def reversal_factor(step, nsteps_period):
+    """cos(pi t / T) at time level t = step (in steps): the flow reverses at
+    T/2 and the exact solution returns to the initial state at t = T."""
+    return math.cos(math.pi * step / nsteps_period)
+
+
+def slotted_cylinders(np_, mp, lp, noise=1e-4, seed=2, ih=1):
+    """Sign-changing field on the unit cube: two slotted cylinders with axes
+    along z (0.2 <= z <= 0.8), radius 0.15, value +1 centred at (0.5, 0.3)
+    and -1 centred at (0.5, 0.7), each with a slot of width 0.06 cut from
+    0.05 inside the centre out to the rim, facing away from the other
+    cylinder; zero background, plus seeded uniform noise in [-noise, noise]
+    everywhere."""
+    rng = np.random.default_rng(seed)
+    xs = (np.arange(1, np_ + 1) - 0.5) / np_
+    ys = (np.arange(1, mp + 1) - 0.5) / mp
+    zs = (np.arange(1, lp + 1) - 0.5) / lp
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    inz = (Z >= 0.2) & (Z <= 0.8)
+    f = np.zeros(Z.shape)
+    for yc, val, sgn in ((0.3, 1.0, -1.0), (0.7, -1.0, 1.0)):
+        disc = (X - 0.5) ** 2 + (Y - yc) ** 2 <= 0.15 ** 2
+        slot = (np.abs(X - 0.5) < 0.03) & (sgn * (Y - yc) > -0.05)
+        f[inz & disc & ~slot] = val
+    f += rng.uniform(-noise, noise, f.shape)
+    out = np.zeros(shape(np_, mp, lp, ih))
+    out[ih:ih + lp, ih:ih + mp, ih:ih + np_] = f
+    return out

@@ -41,10 +41,32 @@ def parse():
     ap.add_argument("--size", type=int, nargs=3, default=(512, 512, 256), metavar=("NP", "MP", "LP"))
     ap.add_argument("--dtype", choices=("f64", "f32"), default="f64")
     ap.add_argument("--path", choices=("auto", "fused", "stages"), default="auto")
-    ap.add_argument("--cpu-steps", type=int, default=1,
+    ap.add_argument("--cpu-steps", type=int, default=2,
                     help="steps of the CPU baseline sample (0 disables it)")
     ap.add_argument("--no-check", action="store_true", help="skip the fused-vs-stages check")
+    ap.add_argument("--no-unfused", action="store_true",
+                    help="skip timing the per-stage (unfused) path beside the fused one")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of one field (counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(a):
+    """A plain copy of one scalar field of the workload, 5 times: the HBM
+    counters of this kernel calibrate FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    np_, mp_, lp_ = a.size
+    n = (np_ + 2) * (mp_ + 2) * (lp_ + 2)
+    tdt = torch.float64 if a.dtype == "f64" else torch.float32
+    src = torch.rand(n, dtype=tdt, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": src.numel() * src.element_size(),
+                      "copy_probe_bytes_written": dst.numel() * dst.element_size(),
+                      "launches": 5}), flush=True)
 
 
 def main():
@@ -53,6 +75,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the step shards by 3-D domain decomposition, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe(a)
+        return
     import torch
 
     from advectionhpctester_amd import mpdata, testdata
@@ -74,16 +99,20 @@ def main():
             dom.step_oop(bufs[s & 1], bufs[(s + 1) & 1], *args)
         return start + n
 
-    s = run(a.warmup, 0)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record()
-    s = run(a.steps, s)
-    ev1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = ev0.elapsed_time(ev1)
+    def timed(s):
+        """Warm up, then time a.steps steps with HIP events on the stream the
+        kernels are launched on (torch's current stream)."""
+        s = run(a.warmup, s)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record()
+        s = run(a.steps, s)
+        ev1.record()
+        torch.cuda.synchronize()
+        return s, ev0.elapsed_time(ev1), time.perf_counter() - t0
+
+    s, gpu_ms, wall = timed(0)
     cells = np_ * mp_ * lp_
     ms_step = gpu_ms / max(a.steps, 1)
     value = cells * a.steps / (gpu_ms / 1e3)
@@ -111,6 +140,14 @@ def main():
         checked = bool(torch.equal(xf, xs))
         del xf, xs
 
+    unfused = None
+    if not a.no_unfused and a.path != "stages":
+        dom.set_path(mpdata.PATH_STAGES)
+        s, ms_unf, _ = timed(s)
+        unfused = {"ms_per_step": ms_unf / max(a.steps, 1),
+                   "value": cells * a.steps / (ms_unf / 1e3),
+                   "hbm_alg_GBs": alg_bytes / (ms_unf / max(a.steps, 1) / 1e3) / 1e9}
+
     cpu = None
     if a.cpu_steps > 0:
         cpu = cpu_baseline(case, (np_, mp_, lp_, 1), a.dtype, a.cpu_steps)
@@ -137,8 +174,11 @@ def main():
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_step": alg_bytes},
+                     "alg_bytes_per_step": alg_bytes,
+                     "duration": "HIP events around the timed steps on the launch stream; "
+                                 "a step is the fused kernel plus a halo-copy kernel"},
         "cpu_baseline": cpu,
+        "unfused_stages": unfused,
         "checked_fused_equals_stages": checked,
         "wall_s_timed": wall,
     }

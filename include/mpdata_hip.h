@@ -241,6 +241,18 @@ int mpdata_norm_sums_f64(mpdata_ctx *ctx, const double *x, const double *x0, dou
 int mpdata_norm_sums_f32(mpdata_ctx *ctx, const float *x, const float *x0, double *out,
                          void *stream);
 
+/* ---- device memory for hosts without a GPU allocator ------------------------
+ * The reference declares its fields MANAGED and its scratch DEVICE through
+ * CUDA Fortran attributes (advection/include/defines.inc:1-11).  A Fortran
+ * host that links this library allocates its fields here instead and passes
+ * the returned device addresses to the entry points above. */
+int mpdata_dev_alloc(long long bytes, void **ptr);  /* zero-filled */
+int mpdata_dev_free(void *ptr);
+/* kind: 0 host->device, 1 device->host, 2 device->device.  Synchronous. */
+int mpdata_copy(void *dst, const void *src, long long bytes, int kind);
+/* Waits for all work queued on 'stream' (NULL: the whole device). */
+int mpdata_sync(void *stream);
+
 #ifdef __cplusplus
 }
 #endif
