@@ -60,6 +60,23 @@ def _load():
     lib.mpdata_ctx_scratch_bytes.argtypes = [_P]
     lib.mpdata_ctx_scratch_bytes.restype = _LL
     lib.upwind_c_timeloop.argtypes = [_P] * 6 + [_I] * 5
+    IP = ctypes.POINTER(_I)
+    lib.mpdata_ens_create.argtypes = [_I, IP, IP, IP, _I, _I, ctypes.POINTER(_P)]
+    lib.mpdata_ens_destroy.argtypes = [_P]
+    lib.mpdata_ens_set_fields_f64.argtypes = [_P] + [ctypes.POINTER(_P)] * 9
+    lib.mpdata_ens_set_fields_f32.argtypes = [_P] + [ctypes.POINTER(_P)] * 9
+    lib.mpdata_ens_nconfigs.argtypes = []
+    lib.mpdata_ens_config_name.argtypes = [_I]
+    lib.mpdata_ens_config_name.restype = ctypes.c_char_p
+    lib.mpdata_ens_set_tiling.argtypes = [_P, _I, _I]
+    lib.mpdata_ens_info.argtypes = [_P, IP, IP, IP]
+    lib.mpdata_ens_autotune.argtypes = [_P, _I, _P, IP, IP, ctypes.POINTER(_D)]
+    lib.mpdata_ens_step.argtypes = [_P, _I, _P]
+    lib.mpdata_ens_run.argtypes = [_P, _I, _I, _P]
+    lib.mpdata_dev_alloc.argtypes = [_LL, ctypes.POINTER(_P)]
+    lib.mpdata_dev_free.argtypes = [_P]
+    lib.mpdata_copy.argtypes = [_P, _P, _LL, _I]
+    lib.mpdata_sync.argtypes = [_P]
     for sfx, real in (("_f64", _D), ("_f32", _F)):
         for name, args in _proto(real).items():
             fn = getattr(lib, name + sfx)

@@ -12,12 +12,15 @@
 #include "../../include/mpdata_hip.h"
 #include "mpdata_fused.hpp"
 #include "mpdata_stages.hpp"
+#include "mpdata_status.hpp"
 
 using namespace mpdata;
 
-namespace {
+namespace mpdata {
 
+namespace {
 thread_local std::string g_last_error;
+}  // namespace
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -33,13 +36,9 @@ int hip_fail(hipError_t e, const char* where) {
     return fail(MPDATA_EHIP, "%s: %s", where, hipGetErrorString(e));
 }
 
-#define HIPCHK(call)                                   \
-    do {                                               \
-        hipError_t e_ = (call);                        \
-        if (e_ != hipSuccess) return hip_fail(e_, #call); \
-    } while (0)
+const char* last_error() { return g_last_error.c_str(); }
 
-}  // namespace
+}  // namespace mpdata
 
 struct mpdata_ctx {
     Geo g;
@@ -400,7 +399,7 @@ extern "C" {
 
 const char* mpdata_version(void) { return "mpdata_hip 0.1 (gfx950)"; }
 
-const char* mpdata_last_error(void) { return g_last_error.c_str(); }
+const char* mpdata_last_error(void) { return mpdata::last_error(); }
 
 int mpdata_ctx_create(int np, int mp, int lp, int ih, int ibcx, int ibcy, int ibcz,
                       int real_bytes, mpdata_ctx** out) {

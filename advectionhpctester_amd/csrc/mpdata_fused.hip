@@ -338,30 +338,43 @@ struct XSrc {  // where one sx cell comes from: x, bcx, bcy or nothing
     int base, stride;
 };
 
-template <typename T, int RX, int RY, int MINB>
-__global__ void __launch_bounds__(RX * RY, MINB)
-k_fused_gauge2(Geo g, Off o, int ntx, int nty, int nchunk, int clen,
-               const T* __restrict__ x, const T* __restrict__ u1, const T* __restrict__ u2,
-               const T* __restrict__ u3, const T* __restrict__ rhr, const T* __restrict__ h,
-               const T* __restrict__ bcx, const T* __restrict__ bcy, T* __restrict__ xo) {
+// LDS of one workgroup of the fixed-position kernel.
+template <typename T, int RX, int RY>
+struct FusedLds {
+    T sx[RY + 2][RX + 2];
+    T sxa[2][RY][RX];
+    T sv1[2][RY][RX];
+    T sv2[2][RY][RX];
+    T scp[3][RY][RX];
+    T scn[3][RY][RX];
+};
+
+// One work item: output tile `tile` (ntx tiles per row) over the planes of
+// k-chunk `chunk` (clen planes).  Every argument is uniform over the
+// workgroup, so all threads pass the same barriers.
+template <typename T, int RX, int RY>
+__device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, int clen, int tile,
+                                           int chunk, const T* __restrict__ x, const T* __restrict__ u1,
+                                           const T* __restrict__ u2, const T* __restrict__ u3,
+                                           const T* __restrict__ rhr, const T* __restrict__ h,
+                                           const T* __restrict__ bcx, const T* __restrict__ bcy,
+                                           T* __restrict__ xo,
+                                           FusedLds<T, RX, RY>& L) {
     constexpr int NT = RX * RY, TX = RX - 4, TY = RY - 4;
     constexpr int SW = RX + 2, SH = RY + 2, NRING = 2 * SW + 2 * (SH - 2);
     static_assert(NRING <= NT, "x ring must fit one extra slot per thread");
-    __shared__ T sx[SH][SW];
-    __shared__ T sxa[2][RY][RX];
-    __shared__ T sv1[2][RY][RX];
-    __shared__ T sv2[2][RY][RX];
-    __shared__ T scp[3][RY][RX];
-    __shared__ T scn[3][RY][RX];
-
+    auto& sx = L.sx;
+    auto& sxa = L.sxa;
+    auto& sv1 = L.sv1;
+    auto& sv2 = L.sv2;
+    auto& scp = L.scp;
+    auto& scn = L.scn;
     const int np = g.np, mp = g.mp, lp = g.lp;
     const int t = threadIdx.x;
     const int pi = t % RX, pj = t / RX;  // region index
     const int li = pi - 2, lj = pj - 2;  // tile-local position
     const T ep = Eps<T>::v;
     using hw::donor; using hw::gflux; using hw::max2; using hw::min2; using hw::pp; using hw::pn;
-    const int ntile = ntx * nty;
-    const int nitem = ntile * nchunk;
     const int SXY = o.sxy, FXY = o.fxy, YXY = o.yxy;
 
     // ring slot of the sx plane handled by this thread (if t < NRING)
@@ -383,6 +396,267 @@ k_fused_gauge2(Geo g, Off o, int ntx, int nty, int nchunk, int clen,
     auto kc = [&](int k) { return k < 1 ? 1 : (k > lp ? lp : k); };        // cell planes
     auto fc = [&](int k) { return k < 1 ? 1 : (k > lp + 1 ? lp + 1 : k); };  // z-face planes
 
+
+    const int i0 = 1 + (tile % ntx) * TX, j0 = 1 + (tile / ntx) * TY;
+    const int k0 = 1 + chunk * clen, k1 = min(lp, k0 + clen - 1);
+    const int gi = i0 + li, gj = j0 + lj;
+    const bool in_dom = gi >= 1 && gi <= np && gj >= 1 && gj <= mp;
+    // roles of this position
+    const bool do_v1 = in_dom && gi >= 2 && li >= -1 && lj >= -1 && lj <= TY;
+    const bool do_v2 = in_dom && gj >= 2 && li >= -1 && li <= TX && lj >= -1;
+    const bool do_c = in_dom && li >= -1 && li <= TX && lj >= -1 && lj <= TY;
+    const bool do_d = in_dom && li >= 0 && li < TX && lj >= 0 && lj < TY;
+    // upwind source cell: itself in the domain, the mirror image on the
+    // one-cell halo ring (cp_scnd_last_to_halo_xy_fulld), clamped beyond
+    const int ai = clampi(mirror(gi, np), 1, np), aj = clampi(mirror(gj, mp), 1, mp);
+    const int sai = ai - i0 + 3, saj = aj - j0 + 3;  // its sx coordinates
+    const int bs = o.s(ai, aj, 0), b1 = o.f1(ai, aj, 0), b2 = o.f2(ai, aj, 0);
+    const XSrc<T> own = in_dom ? XSrc<T>{x, bs, SXY} : xsrc(gi, gj);
+    const XSrc<T> ring = t < NRING ? xsrc(i0 + rsi - 3, j0 + rsj - 3) : XSrc<T>{nullptr, 0, 0};
+    // x-neighbour availability for the in-plane max/min (edge copies
+    // duplicate the centre and do not change it)
+    const bool nxm = gi > 1, nxp = gi < np, nym = gj > 1, nyp = gj < mp;
+
+    const int Ks = max(1, k0 - 2), Ke = k1 + 2;
+    const int Alast = min(lp, k1 + 2);
+    // register queues of the source column; during step K they hold
+    //   xq1..3 = x(K-1..K+1), u1a0/1 = u1(i,K-1..K), u1b0/1 = u1(i+1,K-1..K),
+    //   u2a/u2b likewise at j and j+1, u3q0..2 = faces K-1..K+1,
+    //   hq0..2, rq0..2 = h, rhr(K-2..K), xaq1..2 = xant(K-2..K-1),
+    //   s1q/s2q = pair sums of planes K-2..K-1, v3q1..2 = V3 faces K-2..K-1,
+    //   xmx0/xmn0 = bounds of x for the limiter on plane K-1
+    T xq1 = x[bs + SXY * kc(Ks - 1)], xq2 = x[bs + SXY * kc(Ks)], xq3 = x[bs + SXY * kc(Ks + 1)];
+    T u1a0 = u1[b1 + FXY * kc(Ks - 1)], u1a1 = u1[b1 + FXY * kc(Ks)];
+    T u1b0 = u1[b1 + 1 + FXY * kc(Ks - 1)], u1b1 = u1[b1 + 1 + FXY * kc(Ks)];
+    T u2a0 = u2[b2 + YXY * kc(Ks - 1)], u2a1 = u2[b2 + YXY * kc(Ks)];
+    T u2b0 = u2[b2 + o.yx + YXY * kc(Ks - 1)], u2b1 = u2[b2 + o.yx + YXY * kc(Ks)];
+    T u3q0 = u3[bs + SXY * fc(Ks - 1)], u3q1 = u3[bs + SXY * fc(Ks)], u3q2 = u3[bs + SXY * fc(Ks + 1)];
+    T hq0 = h[bs + SXY * kc(Ks - 2)], hq1 = h[bs + SXY * kc(Ks - 1)], hq2 = h[bs + SXY * kc(Ks)];
+    T rq0 = rhr[bs + SXY * kc(Ks - 2)], rq1 = rhr[bs + SXY * kc(Ks - 1)], rq2 = rhr[bs + SXY * kc(Ks)];
+    T xaq1 = T(0), xaq2 = T(0);
+    T s1q1 = T(0), s1q2 = T(0), s2q1 = T(0), s2q2 = T(0);
+    T v3q1 = T(0), v3q2 = T(0);
+    T xmx0 = T(0), xmn0 = T(0), xmx1 = T(0), xmn1 = T(0);  // x in-plane max/min, planes K-1, K
+
+    for (int K = Ks; K <= Ke; ++K) {
+        const bool A_on = K <= Alast;
+        // x ring cell of plane K (own cell comes from the queue)
+        const T xr = A_on ? ld(ring, K) : T(0);
+        // neighbour-column inputs of V1/V2 on plane P=K-1, loaded before
+        // the barriers so that their latency overlaps stage A
+        const int P = K - 1;
+        const bool BC_on = P >= max(1, k0 - 1) && P <= min(lp, k1 + 1);
+        const bool zt = !(P == 1 || P == lp);
+        T nv1_h = T(0), nv1_u2a = T(0), nv1_u2b = T(0), nv1_u3a = T(0), nv1_u3b = T(0);
+        T nv2_h = T(0), nv2_u1a = T(0), nv2_u1b = T(0), nv2_u3a = T(0), nv2_u3b = T(0);
+        if (BC_on && do_v1) {
+            const int cm = bs - 1 + SXY * P;  // (gi-1, gj, P)
+            const int c2 = b2 - 1 + YXY * P;
+            nv1_h = h[cm];
+            nv1_u2a = u2[c2];
+            nv1_u2b = u2[c2 + o.yx];
+            if (zt) { nv1_u3a = u3[cm]; nv1_u3b = u3[cm + SXY]; }
+        }
+        if (BC_on && do_v2) {
+            const int cm = bs - o.sx + SXY * P;  // (gi, gj-1, P)
+            const int c1 = b1 - o.fx + FXY * P;
+            nv2_h = h[cm];
+            nv2_u1a = u1[c1];
+            nv2_u1b = u1[c1 + 1];
+            if (zt) { nv2_u3a = u3[cm]; nv2_u3b = u3[cm + SXY]; }
+        }
+
+        __syncthreads();  // previous step's LDS readers are done
+        // ---- A: xant on plane K ------------------------------------------
+        if (A_on) {
+            sx[pj + 1][pi + 1] = in_dom ? xq2 : ld(own, K);
+            if (t < NRING) sx[rsj][rsi] = xr;
+        }
+        __syncthreads();
+        T xa_new = T(0);
+        if (A_on) {
+            const bool gnd = K == 1, sky = K == lp;
+            T hinv = T(1) / hq2;
+            T xc = xq2;
+            T f1p = donor(xc, sx[saj][sai + 1], u1b1);
+            T f1 = donor(sx[saj][sai - 1], xc, u1a1);
+            T f2p = donor(xc, sx[saj + 1][sai], u2b1);
+            T f2 = donor(sx[saj - 1][sai], xc, u2a1);
+            T f3p, f3;
+            if (gnd) {
+                f3p = donor(xc, xq3, u3q2);
+                f3 = -f3p;
+            } else if (sky) {
+                f3 = donor(xq1, xc, u3q1);
+                f3p = -f3;
+            } else {
+                f3p = donor(xc, xq3, u3q2);
+                f3 = donor(xq1, xc, u3q1);
+            }
+            xa_new = rq2 * (xc - (f1p - f1 + f2p - f2 + f3p - f3) * hinv);
+            sxa[K & 1][pj][pi] = xa_new;
+            // bounds of x for the limiter on plane K: the 7-point stencil,
+            // where out-of-domain neighbours are edge copies of the centre
+            // (cp_last_to_halo_xy) and leave max/min unchanged
+            T m = xc, n = xc;
+            if (nxm) { T v = sx[pj + 1][pi]; m = max2(m, v); n = min2(n, v); }
+            if (nxp) { T v = sx[pj + 1][pi + 2]; m = max2(m, v); n = min2(n, v); }
+            if (nym) { T v = sx[pj][pi + 1]; m = max2(m, v); n = min2(n, v); }
+            if (nyp) { T v = sx[pj + 2][pi + 1]; m = max2(m, v); n = min2(n, v); }
+            if (!gnd) { m = max2(m, xq1); n = min2(n, xq1); }
+            if (!sky) { m = max2(m, xq3); n = min2(n, xq3); }
+            xmx1 = m;
+            xmn1 = n;
+        }
+        __syncthreads();
+#define XA(pl, a, b) sxa[(pl) & 1][(b) + 2][(a) + 2]
+        // ---- B: pair sums, V1/V2 on plane P=K-1, V3 on face K --------------
+        T s1n = T(0), s2n = T(0), v3n = T(0);
+        if (A_on) {
+            if (do_v1) s1n = xa_new + XA(K, li - 1, lj);
+            if (do_v2) s2n = xa_new + XA(K, li, lj - 1);
+            if (do_c && K >= 2 && K >= k0 - 1) {
+                T hmz = T(1) / (T(0.5) * (hq1 + hq2));
+                T s1 = u1a0 + u1a1 + u1b1 + u1b0;
+                T s2 = u2a0 + u2b0 + u2b1 + u2a1;
+                T cr = s1 * rat4(XA(K - 1, li - 1, lj), XA(K, li - 1, lj), XA(K - 1, li + 1, lj),
+                                 XA(K, li + 1, lj)) +
+                       s2 * rat4(XA(K - 1, li, lj - 1), XA(K, li, lj - 1), XA(K - 1, li, lj + 1),
+                                 XA(K, li, lj + 1));
+                T a = u3q1;
+                v3n = vdyf(xaq2, xa_new, a, hmz) - T(0.125) * a * hmz * (cr);
+            }
+        }
+        if (BC_on) {
+            const bool zterms = !(P == 1 || P == lp);
+            T val1 = T(0), val2 = T(0);
+            if (do_v1) {
+                T hmx = T(1) / (T(0.5) * (nv1_h + hq1));
+                T s2 = nv1_u2a + nv1_u2b + u2b0 + u2a0;
+                T cr = s2 * rat4(XA(P, li - 1, lj - 1), XA(P, li, lj - 1), XA(P, li - 1, lj + 1),
+                                 XA(P, li, lj + 1));
+                if (zterms) {
+                    T s3 = nv1_u3a + nv1_u3b + u3q1 + u3q0;
+                    cr = cr + s3 * ((s1n - s1q1) * T(0.25));
+                }
+                T a = u1a0;
+                val1 = vdyf(XA(P, li - 1, lj), XA(P, li, lj), a, hmx) - T(0.125) * a * hmx * (cr);
+            }
+            if (do_v2) {
+                T hmy = T(1) / (T(0.5) * (nv2_h + hq1));
+                T s1 = nv2_u1a + u1a0 + u1b0 + nv2_u1b;
+                T cr = s1 * rat4(XA(P, li - 1, lj - 1), XA(P, li - 1, lj), XA(P, li + 1, lj - 1),
+                                 XA(P, li + 1, lj));
+                if (zterms) {
+                    T s3 = nv2_u3a + u3q0 + u3q1 + nv2_u3b;
+                    cr = cr + s3 * ((s2n - s2q1) * T(0.25));
+                }
+                T a = u2a0;
+                val2 = vdyf(XA(P, li, lj - 1), XA(P, li, lj), a, hmy) - T(0.125) * a * hmy * (cr);
+            }
+            sv1[P & 1][pj][pi] = val1;
+            sv2[P & 1][pj][pi] = val2;
+        }
+        // prefetch the source-column inputs of plane K+1 (consumed at the
+        // end of this step)
+        const int Kn = K + 1;
+        const T n_x = x[bs + SXY * kc(Kn + 1)];
+        const T n_u1a = u1[b1 + FXY * kc(Kn)], n_u1b = u1[b1 + 1 + FXY * kc(Kn)];
+        const T n_u2a = u2[b2 + YXY * kc(Kn)], n_u2b = u2[b2 + o.yx + YXY * kc(Kn)];
+        const T n_u3 = u3[bs + SXY * fc(Kn + 1)];
+        const T n_h = h[bs + SXY * kc(Kn)], n_r = rhr[bs + SXY * kc(Kn)];
+        __syncthreads();
+        // ---- C: cp, cn on plane P ------------------------------------------
+        if (BC_on) {
+            const bool gnd = P == 1, sky = P == lp;
+            T rp = T(0), rn = T(0);
+            if (do_c) {
+                T mxo = xmx0, mno = xmn0;
+                T mx = XA(P, li - 1, lj), mn = mx;
+#define ACC(v, lo, hi) { T t_ = (v); lo = min2(lo, t_); hi = max2(hi, t_); }
+                ACC(xaq2, mn, mx) ACC(XA(P, li + 1, lj), mn, mx)
+                ACC(XA(P, li, lj - 1), mn, mx) ACC(XA(P, li, lj + 1), mn, mx)
+                if (!gnd) ACC(xaq1, mn, mx)
+                if (!sky) ACC(xa_new, mn, mx)
+#undef ACC
+                mx = max2(mx, mxo);
+                mn = min2(mn, mno);
+                T a1p = sv1[P & 1][pj][pi + 1], a1 = sv1[P & 1][pj][pi];
+                T a2p = sv2[P & 1][pj + 1][pi], a2 = sv2[P & 1][pj][pi];
+                T a3p, a3;
+                if (gnd) { a3p = v3n; a3 = -a3p; }
+                else if (sky) { a3 = v3q2; a3p = -a3; }
+                else { a3p = v3n; a3 = v3q2; }
+                T hc = hq1;
+                rp = (mx - xaq2) * hc / (pn(a1p) + pp(a1) + pn(a2p) + pp(a2) + pn(a3p) + pp(a3) + ep);
+                rn = (xaq2 - mn) * hc / (pp(a1p) + pn(a1) + pp(a2p) + pn(a2) + pp(a3p) + pn(a3) + ep);
+            }
+            scp[P % 3][pj][pi] = rp;
+            scn[P % 3][pj][pi] = rn;
+        }
+        // ---- D: the new x on plane Q=K-2 (neighbour data of plane Q was
+        //      written in the previous step; own cp/cn of Q+1 just now) ----
+        const int Q = K - 2;
+        if (do_d && Q >= k0 && Q <= k1) {
+            const bool gnd = Q == 1, sky = Q == lp;
+#define CP(pl, a, b) scp[(pl) % 3][(b) + 2][(a) + 2]
+#define CN(pl, a, b) scn[(pl) % 3][(b) + 2][(a) + 2]
+            const T v1c = sv1[Q & 1][pj][pi], v1p = sv1[Q & 1][pj][pi + 1];
+            const T v2c = sv2[Q & 1][pj][pi], v2p = sv2[Q & 1][pj + 1][pi];
+            T F1 = gflux(v1c, CP(Q, li, lj), CN(Q, li - 1, lj), CP(Q, li - 1, lj), CN(Q, li, lj));
+            T F1p = gflux(v1p, CP(Q, li + 1, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q, li + 1, lj));
+            T F2 = gflux(v2c, CP(Q, li, lj), CN(Q, li, lj - 1), CP(Q, li, lj - 1), CN(Q, li, lj));
+            T F2p = gflux(v2p, CP(Q, li, lj + 1), CN(Q, li, lj), CP(Q, li, lj), CN(Q, li, lj + 1));
+            T F3, F3p;
+            if (gnd) {
+                const T v = v3q2;
+                F3 = gflux(-v, CP(Q, li, lj), CN(Q + 1, li, lj), CP(Q + 1, li, lj), CN(Q, li, lj));
+                F3p = gflux(v, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
+            } else if (sky) {
+                const T v = v3q1;
+                F3 = gflux(v, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
+                F3p = gflux(-v, CP(Q - 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q - 1, li, lj));
+            } else {
+                F3 = gflux(v3q1, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
+                F3p = gflux(v3q2, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
+            }
+#undef CP
+#undef CN
+            const int c = bs + SXY * Q;
+            const T r = rq0;
+            xo[c] = r * (xaq1 / r - (F1p - F1 + F2p - F2 + F3p - F3) / hq0);
+            if (gi == 1 || gi == np || gj == 1 || gj == mp) {
+                const T xe = x[c];
+                if (gi == 1) xo[c - 1] = xe;
+                if (gi == np) xo[c + 1] = xe;
+                if (gj == 1) xo[c - o.sx] = xe;
+                if (gj == mp) xo[c + o.sx] = xe;
+            }
+        }
+#undef XA
+        // ---- rotate the queues to plane K+1 ----------------------------------
+        xq1 = xq2; xq2 = xq3; xq3 = n_x;
+        u1a0 = u1a1; u1a1 = n_u1a; u1b0 = u1b1; u1b1 = n_u1b;
+        u2a0 = u2a1; u2a1 = n_u2a; u2b0 = u2b1; u2b1 = n_u2b;
+        u3q0 = u3q1; u3q1 = u3q2; u3q2 = n_u3;
+        hq0 = hq1; hq1 = hq2; hq2 = n_h;
+        rq0 = rq1; rq1 = rq2; rq2 = n_r;
+        xaq1 = xaq2; xaq2 = xa_new;
+        s1q1 = s1q2; s1q2 = s1n;
+        s2q1 = s2q2; s2q2 = s2n;
+        v3q1 = v3q2; v3q2 = v3n;
+        xmx0 = xmx1; xmn0 = xmn1;
+    }
+}
+
+template <typename T, int RX, int RY, int MINB>
+__global__ void __launch_bounds__(RX * RY, MINB)
+k_fused_gauge2(Geo g, Off o, int ntx, int nty, int nchunk, int clen,
+               const T* __restrict__ x, const T* __restrict__ u1, const T* __restrict__ u2,
+               const T* __restrict__ u3, const T* __restrict__ rhr, const T* __restrict__ h,
+               const T* __restrict__ bcx, const T* __restrict__ bcy, T* __restrict__ xo) {
+    __shared__ FusedLds<T, RX, RY> L;
+    const int ntile = ntx * nty;
     // Work-item order: workgroups are dealt to the XCDs round-robin
     // (blockIdx % NXCD), each with its own L2.  Item w of workgroup b maps so
     // that the workgroups of one XCD process a contiguous run of tiles
@@ -401,261 +675,46 @@ k_fused_gauge2(Geo g, Off o, int ntx, int nty, int nchunk, int clen,
         if (chunk >= nchunk) break;
         const int tile = xcd * tiles_per_xcd + tix;
         if (tile >= ntile) continue;
-        const int i0 = 1 + (tile % ntx) * TX, j0 = 1 + (tile / ntx) * TY;
-        const int k0 = 1 + chunk * clen, k1 = min(lp, k0 + clen - 1);
-        const int gi = i0 + li, gj = j0 + lj;
-        const bool in_dom = gi >= 1 && gi <= np && gj >= 1 && gj <= mp;
-        // roles of this position
-        const bool do_v1 = in_dom && gi >= 2 && li >= -1 && lj >= -1 && lj <= TY;
-        const bool do_v2 = in_dom && gj >= 2 && li >= -1 && li <= TX && lj >= -1;
-        const bool do_c = in_dom && li >= -1 && li <= TX && lj >= -1 && lj <= TY;
-        const bool do_d = in_dom && li >= 0 && li < TX && lj >= 0 && lj < TY;
-        // upwind source cell: itself in the domain, the mirror image on the
-        // one-cell halo ring (cp_scnd_last_to_halo_xy_fulld), clamped beyond
-        const int ai = clampi(mirror(gi, np), 1, np), aj = clampi(mirror(gj, mp), 1, mp);
-        const int sai = ai - i0 + 3, saj = aj - j0 + 3;  // its sx coordinates
-        const int bs = o.s(ai, aj, 0), b1 = o.f1(ai, aj, 0), b2 = o.f2(ai, aj, 0);
-        const XSrc<T> own = in_dom ? XSrc<T>{x, bs, SXY} : xsrc(gi, gj);
-        const XSrc<T> ring = t < NRING ? xsrc(i0 + rsi - 3, j0 + rsj - 3) : XSrc<T>{nullptr, 0, 0};
-        // x-neighbour availability for the in-plane max/min (edge copies
-        // duplicate the centre and do not change it)
-        const bool nxm = gi > 1, nxp = gi < np, nym = gj > 1, nyp = gj < mp;
-
-        const int Ks = max(1, k0 - 2), Ke = k1 + 2;
-        const int Alast = min(lp, k1 + 2);
-        // register queues of the source column; during step K they hold
-        //   xq1..3 = x(K-1..K+1), u1a0/1 = u1(i,K-1..K), u1b0/1 = u1(i+1,K-1..K),
-        //   u2a/u2b likewise at j and j+1, u3q0..2 = faces K-1..K+1,
-        //   hq0..2, rq0..2 = h, rhr(K-2..K), xaq1..2 = xant(K-2..K-1),
-        //   s1q/s2q = pair sums of planes K-2..K-1, v3q1..2 = V3 faces K-2..K-1,
-        //   xmx0/xmn0 = bounds of x for the limiter on plane K-1
-        T xq1 = x[bs + SXY * kc(Ks - 1)], xq2 = x[bs + SXY * kc(Ks)], xq3 = x[bs + SXY * kc(Ks + 1)];
-        T u1a0 = u1[b1 + FXY * kc(Ks - 1)], u1a1 = u1[b1 + FXY * kc(Ks)];
-        T u1b0 = u1[b1 + 1 + FXY * kc(Ks - 1)], u1b1 = u1[b1 + 1 + FXY * kc(Ks)];
-        T u2a0 = u2[b2 + YXY * kc(Ks - 1)], u2a1 = u2[b2 + YXY * kc(Ks)];
-        T u2b0 = u2[b2 + o.yx + YXY * kc(Ks - 1)], u2b1 = u2[b2 + o.yx + YXY * kc(Ks)];
-        T u3q0 = u3[bs + SXY * fc(Ks - 1)], u3q1 = u3[bs + SXY * fc(Ks)], u3q2 = u3[bs + SXY * fc(Ks + 1)];
-        T hq0 = h[bs + SXY * kc(Ks - 2)], hq1 = h[bs + SXY * kc(Ks - 1)], hq2 = h[bs + SXY * kc(Ks)];
-        T rq0 = rhr[bs + SXY * kc(Ks - 2)], rq1 = rhr[bs + SXY * kc(Ks - 1)], rq2 = rhr[bs + SXY * kc(Ks)];
-        T xaq1 = T(0), xaq2 = T(0);
-        T s1q1 = T(0), s1q2 = T(0), s2q1 = T(0), s2q2 = T(0);
-        T v3q1 = T(0), v3q2 = T(0);
-        T xmx0 = T(0), xmn0 = T(0), xmx1 = T(0), xmn1 = T(0);  // x in-plane max/min, planes K-1, K
-
-        for (int K = Ks; K <= Ke; ++K) {
-            const bool A_on = K <= Alast;
-            // x ring cell of plane K (own cell comes from the queue)
-            const T xr = A_on ? ld(ring, K) : T(0);
-            // neighbour-column inputs of V1/V2 on plane P=K-1, loaded before
-            // the barriers so that their latency overlaps stage A
-            const int P = K - 1;
-            const bool BC_on = P >= max(1, k0 - 1) && P <= min(lp, k1 + 1);
-            const bool zt = !(P == 1 || P == lp);
-            T nv1_h = T(0), nv1_u2a = T(0), nv1_u2b = T(0), nv1_u3a = T(0), nv1_u3b = T(0);
-            T nv2_h = T(0), nv2_u1a = T(0), nv2_u1b = T(0), nv2_u3a = T(0), nv2_u3b = T(0);
-            if (BC_on && do_v1) {
-                const int cm = bs - 1 + SXY * P;  // (gi-1, gj, P)
-                const int c2 = b2 - 1 + YXY * P;
-                nv1_h = h[cm];
-                nv1_u2a = u2[c2];
-                nv1_u2b = u2[c2 + o.yx];
-                if (zt) { nv1_u3a = u3[cm]; nv1_u3b = u3[cm + SXY]; }
-            }
-            if (BC_on && do_v2) {
-                const int cm = bs - o.sx + SXY * P;  // (gi, gj-1, P)
-                const int c1 = b1 - o.fx + FXY * P;
-                nv2_h = h[cm];
-                nv2_u1a = u1[c1];
-                nv2_u1b = u1[c1 + 1];
-                if (zt) { nv2_u3a = u3[cm]; nv2_u3b = u3[cm + SXY]; }
-            }
-
-            __syncthreads();  // previous step's LDS readers are done
-            // ---- A: xant on plane K ------------------------------------------
-            if (A_on) {
-                sx[pj + 1][pi + 1] = in_dom ? xq2 : ld(own, K);
-                if (t < NRING) sx[rsj][rsi] = xr;
-            }
-            __syncthreads();
-            T xa_new = T(0);
-            if (A_on) {
-                const bool gnd = K == 1, sky = K == lp;
-                T hinv = T(1) / hq2;
-                T xc = xq2;
-                T f1p = donor(xc, sx[saj][sai + 1], u1b1);
-                T f1 = donor(sx[saj][sai - 1], xc, u1a1);
-                T f2p = donor(xc, sx[saj + 1][sai], u2b1);
-                T f2 = donor(sx[saj - 1][sai], xc, u2a1);
-                T f3p, f3;
-                if (gnd) {
-                    f3p = donor(xc, xq3, u3q2);
-                    f3 = -f3p;
-                } else if (sky) {
-                    f3 = donor(xq1, xc, u3q1);
-                    f3p = -f3;
-                } else {
-                    f3p = donor(xc, xq3, u3q2);
-                    f3 = donor(xq1, xc, u3q1);
-                }
-                xa_new = rq2 * (xc - (f1p - f1 + f2p - f2 + f3p - f3) * hinv);
-                sxa[K & 1][pj][pi] = xa_new;
-                // bounds of x for the limiter on plane K: the 7-point stencil,
-                // where out-of-domain neighbours are edge copies of the centre
-                // (cp_last_to_halo_xy) and leave max/min unchanged
-                T m = xc, n = xc;
-                if (nxm) { T v = sx[pj + 1][pi]; m = max2(m, v); n = min2(n, v); }
-                if (nxp) { T v = sx[pj + 1][pi + 2]; m = max2(m, v); n = min2(n, v); }
-                if (nym) { T v = sx[pj][pi + 1]; m = max2(m, v); n = min2(n, v); }
-                if (nyp) { T v = sx[pj + 2][pi + 1]; m = max2(m, v); n = min2(n, v); }
-                if (!gnd) { m = max2(m, xq1); n = min2(n, xq1); }
-                if (!sky) { m = max2(m, xq3); n = min2(n, xq3); }
-                xmx1 = m;
-                xmn1 = n;
-            }
-            __syncthreads();
-#define XA(pl, a, b) sxa[(pl) & 1][(b) + 2][(a) + 2]
-            // ---- B: pair sums, V1/V2 on plane P=K-1, V3 on face K --------------
-            T s1n = T(0), s2n = T(0), v3n = T(0);
-            if (A_on) {
-                if (do_v1) s1n = xa_new + XA(K, li - 1, lj);
-                if (do_v2) s2n = xa_new + XA(K, li, lj - 1);
-                if (do_c && K >= 2 && K >= k0 - 1) {
-                    T hmz = T(1) / (T(0.5) * (hq1 + hq2));
-                    T s1 = u1a0 + u1a1 + u1b1 + u1b0;
-                    T s2 = u2a0 + u2b0 + u2b1 + u2a1;
-                    T cr = s1 * rat4(XA(K - 1, li - 1, lj), XA(K, li - 1, lj), XA(K - 1, li + 1, lj),
-                                     XA(K, li + 1, lj)) +
-                           s2 * rat4(XA(K - 1, li, lj - 1), XA(K, li, lj - 1), XA(K - 1, li, lj + 1),
-                                     XA(K, li, lj + 1));
-                    T a = u3q1;
-                    v3n = vdyf(xaq2, xa_new, a, hmz) - T(0.125) * a * hmz * (cr);
-                }
-            }
-            if (BC_on) {
-                const bool zterms = !(P == 1 || P == lp);
-                T val1 = T(0), val2 = T(0);
-                if (do_v1) {
-                    T hmx = T(1) / (T(0.5) * (nv1_h + hq1));
-                    T s2 = nv1_u2a + nv1_u2b + u2b0 + u2a0;
-                    T cr = s2 * rat4(XA(P, li - 1, lj - 1), XA(P, li, lj - 1), XA(P, li - 1, lj + 1),
-                                     XA(P, li, lj + 1));
-                    if (zterms) {
-                        T s3 = nv1_u3a + nv1_u3b + u3q1 + u3q0;
-                        cr = cr + s3 * ((s1n - s1q1) * T(0.25));
-                    }
-                    T a = u1a0;
-                    val1 = vdyf(XA(P, li - 1, lj), XA(P, li, lj), a, hmx) - T(0.125) * a * hmx * (cr);
-                }
-                if (do_v2) {
-                    T hmy = T(1) / (T(0.5) * (nv2_h + hq1));
-                    T s1 = nv2_u1a + u1a0 + u1b0 + nv2_u1b;
-                    T cr = s1 * rat4(XA(P, li - 1, lj - 1), XA(P, li - 1, lj), XA(P, li + 1, lj - 1),
-                                     XA(P, li + 1, lj));
-                    if (zterms) {
-                        T s3 = nv2_u3a + u3q0 + u3q1 + nv2_u3b;
-                        cr = cr + s3 * ((s2n - s2q1) * T(0.25));
-                    }
-                    T a = u2a0;
-                    val2 = vdyf(XA(P, li, lj - 1), XA(P, li, lj), a, hmy) - T(0.125) * a * hmy * (cr);
-                }
-                sv1[P & 1][pj][pi] = val1;
-                sv2[P & 1][pj][pi] = val2;
-            }
-            // prefetch the source-column inputs of plane K+1 (consumed at the
-            // end of this step)
-            const int Kn = K + 1;
-            const T n_x = x[bs + SXY * kc(Kn + 1)];
-            const T n_u1a = u1[b1 + FXY * kc(Kn)], n_u1b = u1[b1 + 1 + FXY * kc(Kn)];
-            const T n_u2a = u2[b2 + YXY * kc(Kn)], n_u2b = u2[b2 + o.yx + YXY * kc(Kn)];
-            const T n_u3 = u3[bs + SXY * fc(Kn + 1)];
-            const T n_h = h[bs + SXY * kc(Kn)], n_r = rhr[bs + SXY * kc(Kn)];
-            __syncthreads();
-            // ---- C: cp, cn on plane P ------------------------------------------
-            if (BC_on) {
-                const bool gnd = P == 1, sky = P == lp;
-                T rp = T(0), rn = T(0);
-                if (do_c) {
-                    T mxo = xmx0, mno = xmn0;
-                    T mx = XA(P, li - 1, lj), mn = mx;
-#define ACC(v, lo, hi) { T t_ = (v); lo = min2(lo, t_); hi = max2(hi, t_); }
-                    ACC(xaq2, mn, mx) ACC(XA(P, li + 1, lj), mn, mx)
-                    ACC(XA(P, li, lj - 1), mn, mx) ACC(XA(P, li, lj + 1), mn, mx)
-                    if (!gnd) ACC(xaq1, mn, mx)
-                    if (!sky) ACC(xa_new, mn, mx)
-#undef ACC
-                    mx = max2(mx, mxo);
-                    mn = min2(mn, mno);
-                    T a1p = sv1[P & 1][pj][pi + 1], a1 = sv1[P & 1][pj][pi];
-                    T a2p = sv2[P & 1][pj + 1][pi], a2 = sv2[P & 1][pj][pi];
-                    T a3p, a3;
-                    if (gnd) { a3p = v3n; a3 = -a3p; }
-                    else if (sky) { a3 = v3q2; a3p = -a3; }
-                    else { a3p = v3n; a3 = v3q2; }
-                    T hc = hq1;
-                    rp = (mx - xaq2) * hc / (pn(a1p) + pp(a1) + pn(a2p) + pp(a2) + pn(a3p) + pp(a3) + ep);
-                    rn = (xaq2 - mn) * hc / (pp(a1p) + pn(a1) + pp(a2p) + pn(a2) + pp(a3p) + pn(a3) + ep);
-                }
-                scp[P % 3][pj][pi] = rp;
-                scn[P % 3][pj][pi] = rn;
-            }
-            // ---- D: the new x on plane Q=K-2 (neighbour data of plane Q was
-            //      written in the previous step; own cp/cn of Q+1 just now) ----
-            const int Q = K - 2;
-            if (do_d && Q >= k0 && Q <= k1) {
-                const bool gnd = Q == 1, sky = Q == lp;
-#define CP(pl, a, b) scp[(pl) % 3][(b) + 2][(a) + 2]
-#define CN(pl, a, b) scn[(pl) % 3][(b) + 2][(a) + 2]
-                const T v1c = sv1[Q & 1][pj][pi], v1p = sv1[Q & 1][pj][pi + 1];
-                const T v2c = sv2[Q & 1][pj][pi], v2p = sv2[Q & 1][pj + 1][pi];
-                T F1 = gflux(v1c, CP(Q, li, lj), CN(Q, li - 1, lj), CP(Q, li - 1, lj), CN(Q, li, lj));
-                T F1p = gflux(v1p, CP(Q, li + 1, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q, li + 1, lj));
-                T F2 = gflux(v2c, CP(Q, li, lj), CN(Q, li, lj - 1), CP(Q, li, lj - 1), CN(Q, li, lj));
-                T F2p = gflux(v2p, CP(Q, li, lj + 1), CN(Q, li, lj), CP(Q, li, lj), CN(Q, li, lj + 1));
-                T F3, F3p;
-                if (gnd) {
-                    const T v = v3q2;
-                    F3 = gflux(-v, CP(Q, li, lj), CN(Q + 1, li, lj), CP(Q + 1, li, lj), CN(Q, li, lj));
-                    F3p = gflux(v, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
-                } else if (sky) {
-                    const T v = v3q1;
-                    F3 = gflux(v, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
-                    F3p = gflux(-v, CP(Q - 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q - 1, li, lj));
-                } else {
-                    F3 = gflux(v3q1, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
-                    F3p = gflux(v3q2, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
-                }
-#undef CP
-#undef CN
-                const int c = bs + SXY * Q;
-                const T r = rq0;
-                xo[c] = r * (xaq1 / r - (F1p - F1 + F2p - F2 + F3p - F3) / hq0);
-                if (gi == 1 || gi == np || gj == 1 || gj == mp) {
-                    const T xe = x[c];
-                    if (gi == 1) xo[c - 1] = xe;
-                    if (gi == np) xo[c + 1] = xe;
-                    if (gj == 1) xo[c - o.sx] = xe;
-                    if (gj == mp) xo[c + o.sx] = xe;
-                }
-            }
-#undef XA
-            // ---- rotate the queues to plane K+1 ----------------------------------
-            xq1 = xq2; xq2 = xq3; xq3 = n_x;
-            u1a0 = u1a1; u1a1 = n_u1a; u1b0 = u1b1; u1b1 = n_u1b;
-            u2a0 = u2a1; u2a1 = n_u2a; u2b0 = u2b1; u2b1 = n_u2b;
-            u3q0 = u3q1; u3q1 = u3q2; u3q2 = n_u3;
-            hq0 = hq1; hq1 = hq2; hq2 = n_h;
-            rq0 = rq1; rq1 = rq2; rq2 = n_r;
-            xaq1 = xaq2; xaq2 = xa_new;
-            s1q1 = s1q2; s1q2 = s1n;
-            s2q1 = s2q2; s2q2 = s2n;
-            v3q1 = v3q2; v3q2 = v3n;
-            xmx0 = xmx1; xmn0 = xmn1;
-        }
+        fused_item<T, RX, RY>(g, o, ntx, clen, tile, chunk, x, u1, u2, u3, rhr, h, bcx, bcy, xo, L);
     }
 }
 
 // Halo cells of x_out that the step does not define keep their x_in values:
-// the z-halo planes and the x/y halo cells outside the edge copies.
+// the z-halo planes and the x/y halo cells outside the edge copies.  Cell q
+// of the three halo slabs -> its offset, or -1 for an edge copy (written by
+// the step itself).
+__device__ __forceinline__ long long untouched_halo_cell(const Geo& g, long long q,
+                                                         long long nz_slab, long long ny_slab) {
+    const int ih = g.ih;
+    const long long nx = g.np + 2 * ih, ny = g.mp + 2 * ih;
+    int i, j, k;
+    bool copy = true;
+    if (q < nz_slab) {
+        long long r = q % (nx * ny);
+        int layer = (int)(q / (nx * ny));
+        k = layer < ih ? 1 - ih + layer : g.lp + 1 + (layer - ih);
+        i = (int)(r % nx) + 1 - ih;
+        j = (int)(r / nx) + 1 - ih;
+    } else if (q < nz_slab + ny_slab) {
+        long long r = q - nz_slab;
+        i = (int)(r % nx) + 1 - ih;
+        long long tt = r / nx;
+        int row = (int)(tt % (2 * ih));
+        k = (int)(tt / (2 * ih)) + 1;
+        j = row < ih ? 1 - ih + row : g.mp + 1 + (row - ih);
+        copy = !((j == 0 || j == g.mp + 1) && i >= 1 && i <= g.np);
+    } else {
+        long long r = q - nz_slab - ny_slab;
+        int col = (int)(r % (2 * ih));
+        long long tt = r / (2 * ih);
+        j = (int)(tt % g.mp) + 1;
+        k = (int)(tt / g.mp) + 1;
+        i = col < ih ? 1 - ih + col : g.np + 1 + (col - ih);
+        copy = !(i == 0 || i == g.np + 1);
+    }
+    return copy ? off_raw<SCAL>(g, i, j, k) : -1;
+}
+
 template <typename T>
 __global__ void k_copy_untouched_halo(Geo g, const T* __restrict__ x, T* __restrict__ xo) {
     const int ih = g.ih;
@@ -666,35 +725,8 @@ __global__ void k_copy_untouched_halo(Geo g, const T* __restrict__ x, T* __restr
     const long long total = nz_slab + ny_slab + nx_slab;
     for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < total;
          q += (long long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        bool copy = true;
-        if (q < nz_slab) {
-            long long r = q % (nx * ny);
-            int layer = (int)(q / (nx * ny));
-            k = layer < ih ? 1 - ih + layer : g.lp + 1 + (layer - ih);
-            i = (int)(r % nx) + 1 - ih;
-            j = (int)(r / nx) + 1 - ih;
-        } else if (q < nz_slab + ny_slab) {
-            long long r = q - nz_slab;
-            i = (int)(r % nx) + 1 - ih;
-            long long tt = r / nx;
-            int row = (int)(tt % (2 * ih));
-            k = (int)(tt / (2 * ih)) + 1;
-            j = row < ih ? 1 - ih + row : g.mp + 1 + (row - ih);
-            copy = !((j == 0 || j == g.mp + 1) && i >= 1 && i <= g.np);
-        } else {
-            long long r = q - nz_slab - ny_slab;
-            int col = (int)(r % (2 * ih));
-            long long tt = r / (2 * ih);
-            j = (int)(tt % g.mp) + 1;
-            k = (int)(tt / g.mp) + 1;
-            i = col < ih ? 1 - ih + col : g.np + 1 + (col - ih);
-            copy = !(i == 0 || i == g.np + 1);
-        }
-        if (copy) {
-            const long long c = off_raw<SCAL>(g, i, j, k);
-            xo[c] = x[c];
-        }
+        const long long c = untouched_halo_cell(g, q, nz_slab, ny_slab);
+        if (c >= 0) xo[c] = x[c];
     }
 }
 
@@ -741,6 +773,119 @@ hipError_t launch2(const Geo& g, const Off& o, const T* x, T* xout, const T* u1,
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Batched step: many domains (ensemble members, tracers) of different shapes
+// in one persistent launch.  The work items of all members form one list
+// (member by member, each chunk-major over its tiles); the items of a
+// contiguous run of that list go to the workgroups of one XCD.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Member {  // device descriptor of one member
+    Geo g;
+    Off o;
+    int ntx, ntile, clen, item0;
+    const T *x, *u1, *u2, *u3, *rhr, *h, *bcx, *bcy;
+    T* xo;
+};
+
+template <typename T, int RX, int RY, int MINB>
+__global__ void __launch_bounds__(RX * RY, MINB)
+k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem) {
+    __shared__ FusedLds<T, RX, RY> L;
+    const int nxcd = gridDim.x >= NXCD ? NXCD : 1;
+    const int per_xcd = gridDim.x / nxcd;
+    const int xcd = blockIdx.x % nxcd, slot = blockIdx.x / nxcd;
+    if (blockIdx.x >= per_xcd * nxcd) return;
+    const int items_per_xcd = (nitem + nxcd - 1) / nxcd;
+    for (int r = slot; r < items_per_xcd; r += per_xcd) {
+        const int w = xcd * items_per_xcd + r;
+        if (w >= nitem) break;
+        int lo = 0, hi = nmem - 1;  // last member with item0 <= w
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (mem[mid].item0 <= w) lo = mid;
+            else hi = mid - 1;
+        }
+        const Member<T>& m = mem[lo];
+        const int l = w - m.item0;
+        const int chunk = l / m.ntile, tile = l - chunk * m.ntile;
+        fused_item<T, RX, RY>(m.g, m.o, m.ntx, m.clen, tile, chunk, m.x, m.u1, m.u2, m.u3, m.rhr,
+                              m.h, m.bcx, m.bcy, m.xo, L);
+    }
+}
+
+// Halo copies of every member (blockIdx.y = member).
+template <typename T>
+__global__ void k_copy_untouched_halo_batch(const Member<T>* __restrict__ mem) {
+    const Member<T>& m = mem[blockIdx.y];
+    const Geo g = m.g;
+    const int ih = g.ih;
+    const long long nx = g.np + 2 * ih, ny = g.mp + 2 * ih;
+    const long long nz_slab = 2ll * ih * nx * ny;
+    const long long ny_slab = 2ll * ih * nx * g.lp;
+    const long long nx_slab = 2ll * ih * g.mp * g.lp;
+    const long long total = nz_slab + ny_slab + nx_slab;
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < total;
+         q += (long long)gridDim.x * blockDim.x) {
+        const long long c = untouched_halo_cell(g, q, nz_slab, ny_slab);
+        if (c >= 0) m.xo[c] = m.x[c];
+    }
+}
+
+Off make_off(const Geo& g) {
+    Off o;
+    o.ih = g.ih;
+    o.sx = g.np + 2 * g.ih;
+    o.sxy = o.sx * (g.mp + 2 * g.ih);
+    o.fx = g.np + 1 + 2 * g.ih;
+    o.fxy = o.fx * (g.mp + 2 * g.ih);
+    o.yx = g.np + 2 * g.ih;
+    o.yxy = o.yx * (g.mp + 1 + 2 * g.ih);
+    return o;
+}
+
+template <int RX, int RY>
+void tiles_of(const Geo& g, int& ntx, int& nty) {
+    ntx = (g.np + RX - 5) / (RX - 4);
+    nty = (g.mp + RY - 5) / (RY - 4);
+}
+
+template <typename T, int RX, int RY>
+int pack_members(const BatchMember<T>* bm, int n, int clen, Member<T>* out) {
+    long long item = 0;
+    for (int q = 0; q < n; q++) {
+        Member<T>& m = out[q];
+        m.g = bm[q].g;
+        m.o = make_off(m.g);
+        int ntx, nty;
+        tiles_of<RX, RY>(m.g, ntx, nty);
+        m.ntx = ntx;
+        m.ntile = ntx * nty;
+        m.clen = clen < m.g.lp ? clen : m.g.lp;
+        m.item0 = (int)item;
+        m.x = bm[q].x; m.u1 = bm[q].u1; m.u2 = bm[q].u2; m.u3 = bm[q].u3;
+        m.rhr = bm[q].rhr; m.h = bm[q].h; m.bcx = bm[q].bcx; m.bcy = bm[q].bcy;
+        m.xo = bm[q].xo;
+        item += (long long)m.ntile * ((m.g.lp + m.clen - 1) / m.clen);
+        if (item > (1ll << 30)) return -1;
+    }
+    return (int)item;
+}
+
+template <typename T, int RX, int RY, int MINB>
+hipError_t launch_batch(const void* table, int n, int nitem, hipStream_t s) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, k_fused_gauge2_batch<T, RX, RY, MINB>, RX * RY, 0);
+    if (e != hipSuccess) return e;
+    const int grid = std::max(1, std::min(nitem, ncu * std::max(per_cu, 1)));
+    k_fused_gauge2_batch<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>((const Member<T>*)table, n, nitem);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 template <typename T>
@@ -761,14 +906,7 @@ template <typename T>
 hipError_t Fused<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
                           const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
                           hipStream_t s) {
-    Off o;
-    o.ih = g.ih;
-    o.sx = g.np + 2 * g.ih;
-    o.sxy = o.sx * (g.mp + 2 * g.ih);
-    o.fx = g.np + 1 + 2 * g.ih;
-    o.fxy = o.fx * (g.mp + 2 * g.ih);
-    o.yx = g.np + 2 * g.ih;
-    o.yxy = o.yx * (g.mp + 1 + 2 * g.ih);
+    const Off o = make_off(g);
     if (fused_generation() == 1) {
         dim3 grid((g.np + FTX - 1) / FTX, (g.mp + FTY - 1) / FTY, 1);
         k_fused_gauge<T, FTX, FTY><<<grid, FTX * FTY, 0, s>>>(g, o, x, u1, u2, u3, rhr, h, bcx, bcy, xout);
@@ -789,5 +927,69 @@ hipError_t Fused<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const 
 
 template struct Fused<double>;
 template struct Fused<float>;
+
+// ---- batched step ------------------------------------------------------------
+// Tile configurations of the batched kernel: (id, RX, RY, waves per SIMD).
+#define MPDATA_BATCH_CONFIGS(X) \
+    X(0, 32, 16, 2)             \
+    X(1, 32, 16, 4)             \
+    X(2, 64, 16, 4)             \
+    X(3, 16, 32, 2)             \
+    X(4, 64, 8, 2)
+
+template <typename T>
+int FusedBatch<T>::nconfigs() {
+    return 5;
+}
+
+template <typename T>
+const char* FusedBatch<T>::config_name(int cfg) {
+    switch (cfg) {
+#define NAME(c, rx, ry, mb) \
+    case c: return #rx "x" #ry " threads, " #mb " waves/SIMD";
+        MPDATA_BATCH_CONFIGS(NAME)
+#undef NAME
+    }
+    return "?";
+}
+
+template <typename T>
+size_t FusedBatch<T>::table_bytes(int n) {
+    return sizeof(Member<T>) * (size_t)n;
+}
+
+template <typename T>
+int FusedBatch<T>::pack(const BatchMember<T>* m, int n, int cfg, int clen, void* table) {
+    if (clen < 1) return -1;
+    for (int q = 0; q < n; q++)
+        if (!Fused<T>::supports(m[q].g, 1, 2, 1)) return -1;
+    switch (cfg) {
+#define PACK(c, rx, ry, mb) \
+    case c: return pack_members<T, rx, ry>(m, n, clen, (Member<T>*)table);
+        MPDATA_BATCH_CONFIGS(PACK)
+#undef PACK
+    }
+    return -1;
+}
+
+template <typename T>
+hipError_t FusedBatch<T>::step(const void* table, int n, int nitem, int cfg, long long max_halo,
+                               hipStream_t s) {
+    hipError_t e = hipErrorInvalidValue;
+    switch (cfg) {
+#define LAUNCH(c, rx, ry, mb) \
+    case c: e = launch_batch<T, rx, ry, mb>(table, n, nitem, s); break;
+        MPDATA_BATCH_CONFIGS(LAUNCH)
+#undef LAUNCH
+    }
+    if (e != hipSuccess) return e;
+    const int bx = (int)std::min<long long>(64, (max_halo + 255) / 256);
+    k_copy_untouched_halo_batch<T><<<dim3(std::max(bx, 1), n), 256, 0, s>>>((const Member<T>*)table);
+    return hipGetLastError();
+}
+#undef MPDATA_BATCH_CONFIGS
+
+template struct FusedBatch<double>;
+template struct FusedBatch<float>;
 
 }  // namespace mpdata

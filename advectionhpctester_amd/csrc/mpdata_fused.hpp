@@ -18,4 +18,32 @@ struct Fused {
                            hipStream_t s);
 };
 
+// One member of a batched step: its grid and fields (x in, xo out).
+template <typename T>
+struct BatchMember {
+    Geo g;
+    const T *x, *u1, *u2, *u3, *rhr, *h, *bcx, *bcy;
+    T* xo;
+};
+
+// The fused step over many domains of different shapes in one persistent
+// launch (plus one halo-copy launch).  The caller packs a descriptor table
+// on the host (pack), copies it to device memory and launches with it.
+template <typename T>
+struct FusedBatch {
+    // number of tile configurations and a description of each
+    static int nconfigs();
+    static const char* config_name(int cfg);
+    // bytes of the descriptor table for n members
+    static size_t table_bytes(int n);
+    // fills `table` (host memory, table_bytes(n)) for tile configuration cfg
+    // and k-chunk length clen; returns the number of work items, or -1 if a
+    // member is outside the fused kernel's scope or an argument is invalid
+    static int pack(const BatchMember<T>* m, int n, int cfg, int clen, void* table);
+    // one batched step with the device copy of a packed table; max_halo =
+    // the largest number of halo cells of one member
+    static hipError_t step(const void* table, int n, int nitem, int cfg, long long max_halo,
+                           hipStream_t s);
+};
+
 }  // namespace mpdata

@@ -244,3 +244,105 @@ class Domain:
         energy = (abs(s[0] - s[1]) * nmli) ** 0.5 / timetot
         error = (s[2] * nmli) ** 0.5 / timetot
         return energy, error
+
+
+class Ensemble:
+    """Many domains of different shapes (tracers, ensemble members; ih = 1,
+    open walls) advanced together: each step is one persistent launch of the
+    fused gauge kernel over the work items of all members, and a time loop
+    can run as one HIP graph.  Results equal Domain.step_oop on each member
+    bit for bit.  The library keeps raw pointers to the tensors passed to
+    set_fields; this object keeps the tensors alive."""
+
+    def __init__(self, shapes, dtype=torch.float64, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("advectionhpctester_amd needs a GPU (HIP device); none is visible")
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        self.n = len(self.shapes)
+        if dtype == torch.float64:
+            self.sfx, rb = "_f64", 8
+        elif dtype == torch.float32:
+            self.sfx, rb = "_f32", 4
+        else:
+            raise TypeError(f"dtype {dtype}: float64 or float32")
+        self.dtype = dtype
+        self.device = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        arr = ctypes.c_int * self.n
+        self._e = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(LIB.mpdata_ens_create(self.n, arr(*[s[0] for s in self.shapes]),
+                                        arr(*[s[1] for s in self.shapes]),
+                                        arr(*[s[2] for s in self.shapes]), 1, rb,
+                                        ctypes.byref(self._e)))
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "_e", None) and self._e.value:
+            LIB.mpdata_ens_destroy(self._e)
+            self._e = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def set_fields(self, xa, xb, u1, u2, u3, rhr, h, bcx, bcy):
+        """Lists (one tensor per member) in the field layout of Domain."""
+        groups = [xa, xb, u1, u2, u3, rhr, h, bcx, bcy]
+        kinds = [0, 0, 1, 2, 3, 0, 0, None, None]
+        ptrs = []
+        for name, grp, kind in zip("xa xb u1 u2 u3 rhr h bcx bcy".split(), groups, kinds):
+            if len(grp) != self.n:
+                raise ValueError(f"{name}: {len(grp)} tensors for {self.n} members")
+            row = []
+            for q, t in enumerate(grp):
+                np_, mp, lp = self.shapes[q]
+                want = (field_shape(np_, mp, lp, 1, kind) if kind is not None else
+                        ((2, lp, mp) if name == "bcx" else (2, lp, np_)))
+                if tuple(t.shape) != want or t.dtype != self.dtype or not t.is_contiguous() \
+                        or t.device != self.device:
+                    raise ValueError(f"{name}[{q}]: need a contiguous {self.dtype} tensor of shape "
+                                     f"{want} on {self.device}, got {tuple(t.shape)} {t.dtype}")
+                row.append(t.data_ptr())
+            ptrs.append((ctypes.c_void_p * self.n)(*row))
+        check(getattr(LIB, "mpdata_ens_set_fields" + self.sfx)(self._e, *ptrs))
+        self._keep = [list(g) for g in groups]
+
+    @staticmethod
+    def configs():
+        return [LIB.mpdata_ens_config_name(c).decode() for c in range(LIB.mpdata_ens_nconfigs())]
+
+    def set_tiling(self, cfg, clen):
+        check(LIB.mpdata_ens_set_tiling(self._e, cfg, clen))
+
+    def info(self):
+        """(work items per step, tile configuration, k-chunk length)."""
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        check(LIB.mpdata_ens_info(self._e, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def autotune(self, trial_steps=3):
+        """Picks the fastest (configuration, k-chunk length); overwrites xb.
+        Returns (cfg, clen, ms per step)."""
+        cfg, clen, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(LIB.mpdata_ens_autotune(self._e, trial_steps, self._stream(), ctypes.byref(cfg),
+                                      ctypes.byref(clen), ctypes.byref(ms)))
+        return cfg.value, clen.value, ms.value
+
+    def step(self, parity=0):
+        check(LIB.mpdata_ens_step(self._e, parity, self._stream()))
+
+    def run(self, nsteps, graph=True):
+        """nsteps steps from xa; the result is in xa (nsteps even) or xb."""
+        check(LIB.mpdata_ens_run(self._e, nsteps, 1 if graph else 0, self._stream()))

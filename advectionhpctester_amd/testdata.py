@@ -372,3 +372,43 @@ def slotted_cylinders(np_, mp, lp, noise=1e-4, seed=2, ih=1):
     out = np.zeros(shape(np_, mp, lp, ih))
     out[ih:ih + lp, ih:ih + mp, ih:ih + np_] = f
     return out
+
+
+# ---------------------------------------------------------------------------
+# Batched ensemble: many tracers on uneven grids
+# ---------------------------------------------------------------------------
+def ensemble_shapes(nmember=64, seed=4, lo=32, hi=256, step=16):
+    """Per-member (np, mp, lp), each drawn from lo..hi in steps of `step`."""
+    rng = np.random.default_rng(seed)
+    vals = np.arange(lo, hi + 1, step)
+    return [tuple(int(v) for v in rng.choice(vals, 3)) for _ in range(nmember)]
+
+
+def ensemble_member(np_, mp, lp, seed, ih=1, dtype=None, device="cpu"):
+    """One tracer: a Gaussian blob exp(-r^2/R^2) of amplitude 1 on a zero
+    background, R drawn from [4, 16] cells and the centre from the middle
+    40 % of each axis; solid-body rotation about a random axis through the
+    domain centre with max Courant drawn from [0.3, 0.8]; zero inflow,
+    h = rhr = 1.  Returns tensors in the field layout of Domain."""
+    torch = _torch()
+    dtype = dtype or torch.float64
+    rng = np.random.default_rng(seed)
+    radius = rng.uniform(4.0, 16.0)
+    cx, cy, cz = (rng.uniform(0.3, 0.7) * n + 0.5 for n in (np_, mp, lp))
+    cmax = rng.uniform(0.3, 0.8)
+    axis = rng.normal(size=3)
+    axis /= np.linalg.norm(axis)
+    i = torch.arange(1, np_ + 1, dtype=torch.float64, device=device)
+    j = torch.arange(1, mp + 1, dtype=torch.float64, device=device)
+    k = torch.arange(1, lp + 1, dtype=torch.float64, device=device)
+    r2 = ((k - cz) ** 2)[:, None, None] + ((j - cy) ** 2)[None, :, None] + \
+        ((i - cx) ** 2)[None, None, :]
+    x = torch.zeros(shape(np_, mp, lp, ih), dtype=dtype, device=device)
+    x[ih:ih + lp, ih:ih + mp, ih:ih + np_] = torch.exp(-r2 / radius ** 2).to(dtype)
+    ua, va, wa = solid_body_a_grid(np_, mp, lp, axis=tuple(axis), cmax=cmax, ih=ih,
+                                   dtype=torch.float64, device=device)
+    u1, u2, u3 = (a.to(dtype) for a in a_to_c_torch(ua, va, wa, ih))
+    h = torch.ones(shape(np_, mp, lp, ih), dtype=dtype, device=device)
+    return dict(x=x, u1=u1, u2=u2, u3=u3, h=h, rhr=h.clone(),
+                bcx=torch.zeros((2, lp, mp), dtype=dtype, device=device),
+                bcy=torch.zeros((2, lp, np_), dtype=dtype, device=device))

@@ -241,6 +241,47 @@ int mpdata_norm_sums_f64(mpdata_ctx *ctx, const double *x, const double *x0, dou
 int mpdata_norm_sums_f32(mpdata_ctx *ctx, const float *x, const float *x0, double *out,
                          void *stream);
 
+/* ---- ensembles: many domains in one launch ---------------------------------
+ * No counterpart in the reference (its batching is many fields on one grid,
+ * advection_CEtimestep_twostep_GPUBC.f90).  n members of different shapes
+ * (ih = 1, open walls), each advanced by the fused gauge step (2 iterations,
+ * limiter on) in ONE persistent launch per step (plus one halo-copy launch);
+ * a time loop can be captured into a HIP graph and launched once.  Results
+ * are bit-identical to mpdata_step_oop_* on each member. */
+typedef struct mpdata_ens mpdata_ens;
+int mpdata_ens_create(int nmember, const int *np, const int *mp, const int *lp, int ih,
+                      int real_bytes, mpdata_ens **out);
+int mpdata_ens_destroy(mpdata_ens *ens);
+/* Host arrays of nmember device pointers.  x alternates between xa (even
+ * steps read it) and xb; the two must not overlap. */
+int mpdata_ens_set_fields_f64(mpdata_ens *ens, double *const *xa, double *const *xb,
+                              const double *const *u1, const double *const *u2,
+                              const double *const *u3, const double *const *rhr,
+                              const double *const *h, const double *const *bcx,
+                              const double *const *bcy);
+int mpdata_ens_set_fields_f32(mpdata_ens *ens, float *const *xa, float *const *xb,
+                              const float *const *u1, const float *const *u2,
+                              const float *const *u3, const float *const *rhr,
+                              const float *const *h, const float *const *bcx,
+                              const float *const *bcy);
+/* Tile configurations of the batched kernel (0 .. mpdata_ens_nconfigs()-1)
+ * and the k-chunk length (planes one work item marches). */
+int mpdata_ens_nconfigs(void);
+const char *mpdata_ens_config_name(int cfg);
+int mpdata_ens_set_tiling(mpdata_ens *ens, int cfg, int clen);
+int mpdata_ens_info(const mpdata_ens *ens, int *nitem, int *cfg, int *clen);
+/* Times every (configuration, k-chunk length) on trial_steps steps of the
+ * real fields and keeps the fastest; the choice is cached per set of
+ * shapes for the process.  Reads xa and overwrites xb.  Synchronous. */
+int mpdata_ens_autotune(mpdata_ens *ens, int trial_steps, void *stream, int *cfg, int *clen,
+                        double *ms_per_step);
+/* One step: parity 0 xa -> xb, 1 xb -> xa. */
+int mpdata_ens_step(mpdata_ens *ens, int parity, void *stream);
+/* nsteps steps starting from xa (the result is in xa if nsteps is even,
+ * else in xb); use_graph = 1 records the loop once into a HIP graph (kept
+ * until the fields, the tiling or nsteps change) and launches that graph. */
+int mpdata_ens_run(mpdata_ens *ens, int nsteps, int use_graph, void *stream);
+
 /* ---- device memory for hosts without a GPU allocator ------------------------
  * The reference declares its fields MANAGED and its scratch DEVICE through
  * CUDA Fortran attributes (advection/include/defines.inc:1-11).  A Fortran

@@ -97,3 +97,21 @@ def test_product_path_does_not_use_oracle():
             if f.endswith((".py", ".hip", ".hpp", ".h", ".f90", ".F90")) or f == "Makefile":
                 text = open(os.path.join(dirpath, f), errors="replace").read()
                 assert not re.search(r"(import\s+oracle|from\s+oracle|liboracle|\borc_)", text), f
+
+
+def test_ensemble_create_rejects(lib):
+    from advectionhpctester_amd import _lib
+    arr = ctypes.c_int * 2
+    out = ctypes.c_void_p(1)
+    assert lib.mpdata_ens_create(0, arr(8, 8), arr(8, 8), arr(8, 8), 1, 8,
+                                 ctypes.byref(out)) == _lib.MPDATA_EINVAL
+    assert out.value is None
+    assert lib.mpdata_ens_create(2, arr(8, 8), arr(8, 8), arr(8, 8), 2, 8,
+                                 ctypes.byref(out)) == _lib.MPDATA_EUNSUPPORTED
+    assert lib.mpdata_ens_create(2, arr(8, 2), arr(8, 8), arr(8, 8), 1, 8,
+                                 ctypes.byref(out)) == _lib.MPDATA_EINVAL
+    assert "member 1" in lib.mpdata_last_error().decode()
+    assert lib.mpdata_ens_step(None, 0, None) == _lib.MPDATA_EINVAL
+    assert lib.mpdata_ens_nconfigs() >= 1
+    assert lib.mpdata_ens_config_name(0)
+    assert lib.mpdata_ens_config_name(-1) == b""
