@@ -10,6 +10,8 @@
 #include <vector>
 
 #include "../../include/mpdata_hip.h"
+#include "mpdata_ctx.hpp"
+#include "mpdata_dual.hpp"
 #include "mpdata_fused.hpp"
 #include "mpdata_stages.hpp"
 #include "mpdata_status.hpp"
@@ -38,32 +40,6 @@ int hip_fail(hipError_t e, const char* where) {
 
 const char* last_error() { return g_last_error.c_str(); }
 
-}  // namespace mpdata
-
-struct mpdata_ctx {
-    Geo g;
-    int real_bytes;
-    int path;
-    int device;
-    // scratch, allocated on first use
-    void* xant = nullptr;
-    void* psi2 = nullptr;
-    void* xnext = nullptr;      // output buffer of the out-of-place fused step
-    void* v[3] = {nullptr, nullptr, nullptr};
-    void* cp = nullptr;
-    void* cn = nullptr;
-    void* fns[3] = {nullptr, nullptr, nullptr};
-    void* f[3] = {nullptr, nullptr, nullptr};
-    void* ul[3] = {nullptr, nullptr, nullptr};
-    void* adv[3] = {nullptr, nullptr, nullptr};
-    void* red = nullptr;        // reduction partials (device)
-    long long red_bytes = 0;
-    long long held = 0;         // bytes of device scratch
-    std::vector<void*> owned;
-};
-
-namespace {
-
 long long field_len(const Geo& g, int kind) {
     switch (kind) {
         case FACEX: return len_of<FACEX>(g);
@@ -73,9 +49,9 @@ long long field_len(const Geo& g, int kind) {
     }
 }
 
-int ensure(mpdata_ctx* c, void** slot, int kind) {
+int ensure(mpdata_ctx* c, void** slot, int kind, int elem_bytes) {
     if (*slot) return MPDATA_OK;
-    size_t bytes = (size_t)field_len(c->g, kind) * c->real_bytes;
+    size_t bytes = (size_t)field_len(c->g, kind) * elem_bytes;
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
@@ -90,6 +66,18 @@ int ensure(mpdata_ctx* c, void** slot, int kind) {
     c->owned.push_back(p);
     c->held += (long long)bytes;
     return MPDATA_OK;
+}
+
+
+}  // namespace mpdata
+
+
+namespace {
+
+// Scratch set of a real type: the tangent-linear step keeps its own.
+template <typename T>
+Scratch& scr(mpdata_ctx* c) {
+    return is_dual<T>::value ? c->dual : c->prim;
 }
 
 int ensure_red(mpdata_ctx* c, long long bytes) {
@@ -151,24 +139,25 @@ template <typename T>
 int stage_pass(mpdata_ctx* c, int variant, int nonos, const T* u1, const T* u2, const T* u3,
                T* psi, T* x0, T* out, const T* rhr, const T* h, T* ul1, T* ul2, T* ul3,
                hipStream_t s) {
+    Scratch& S = scr<T>(c);
     using St = Stages<T>;
     const Geo& g = c->g;
     const bool stdv = variant == MPDATA_STANDARD;
     int rc;
     for (int d = 0; d < 3; d++)
-        if ((rc = ensure(c, &c->v[d], FACEX + d))) return rc;
+        if ((rc = ensure(c, &S.v[d], FACEX + d, sizeof(T)))) return rc;
     if (nonos) {
-        if ((rc = ensure(c, &c->cp, SCAL))) return rc;
-        if ((rc = ensure(c, &c->cn, SCAL))) return rc;
+        if ((rc = ensure(c, &S.cp, SCAL, sizeof(T)))) return rc;
+        if ((rc = ensure(c, &S.cn, SCAL, sizeof(T)))) return rc;
     }
     if (stdv)
         for (int d = 0; d < 3; d++) {
-            if ((rc = ensure(c, &c->fns[d], FACEX + d))) return rc;
-            if ((rc = ensure(c, &c->f[d], FACEX + d))) return rc;
+            if ((rc = ensure(c, &S.fns[d], FACEX + d, sizeof(T)))) return rc;
+            if ((rc = ensure(c, &S.f[d], FACEX + d, sizeof(T)))) return rc;
         }
-    T *v1 = (T*)c->v[0], *v2 = (T*)c->v[1], *v3 = (T*)c->v[2];
-    T *f1ns = (T*)c->fns[0], *f2ns = (T*)c->fns[1], *f3ns = (T*)c->fns[2];
-    T *cp = (T*)c->cp, *cn = (T*)c->cn;
+    T *v1 = (T*)S.v[0], *v2 = (T*)S.v[1], *v3 = (T*)S.v[2];
+    T *f1ns = (T*)S.fns[0], *f2ns = (T*)S.fns[1], *f3ns = (T*)S.fns[2];
+    T *cp = (T*)S.cp, *cn = (T*)S.cn;
     HIPCHK(St::halo_xy(g, psi, St::MIRROR, 1, s));
     HIPCHK(St::pseudo_vel(g, variant, u1, u2, u3, psi, h, v1, v2, v3, f1ns, f2ns, f3ns, s));
     HIPCHK(St::face_bc(g, v1, v2, v3, s));
@@ -184,7 +173,7 @@ int stage_pass(mpdata_ctx* c, int variant, int nonos, const T* u1, const T* u2, 
         HIPCHK(St::gauge_update(g, nonos, v1, v2, v3, cp, cn, psi, rhr, h, out, ul1, ul2, ul3, s));
         return MPDATA_OK;
     }
-    T *f1 = (T*)c->f[0], *f2 = (T*)c->f[1], *f3 = (T*)c->f[2];
+    T *f1 = (T*)S.f[0], *f2 = (T*)S.f[1], *f3 = (T*)S.f[2];
     HIPCHK(St::std_flux(g, nonos, v1, v2, v3, cp, cn, psi, rhr, f1, f2, f3, ul1, ul2, ul3, s));
     HIPCHK(St::face_bc(g, f1, f2, f3, s));
     if (ul1) HIPCHK(St::face_bc(g, ul1, ul2, ul3, s));
@@ -207,25 +196,26 @@ template <typename T>
 int step_stages(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T* u1,
                 const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
                 hipStream_t s) {
+    Scratch& S = scr<T>(c);
     int rc;
-    if ((rc = ensure(c, &c->xant, SCAL))) return rc;
-    T* xant = (T*)c->xant;
+    if ((rc = ensure(c, &S.xant, SCAL, sizeof(T)))) return rc;
+    T* xant = (T*)S.xant;
     if ((rc = upwind_impl<T>(c, u1, u2, u3, x, xant, rhr, h, bcx, bcy, s))) return rc;
     if (variant == MPDATA_UPWIND || iters <= 1) {
         HIPCHK(Stages<T>::copy_interior(c->g, xant, x, s));
         return MPDATA_OK;
     }
     if (iters > 2) {
-        if ((rc = ensure(c, &c->psi2, SCAL))) return rc;
+        if ((rc = ensure(c, &S.psi2, SCAL, sizeof(T)))) return rc;
         for (int d = 0; d < 3; d++) {
-            if ((rc = ensure(c, &c->ul[d], FACEX + d))) return rc;
-            if ((rc = ensure(c, &c->adv[d], FACEX + d))) return rc;
+            if ((rc = ensure(c, &S.ul[d], FACEX + d, sizeof(T)))) return rc;
+            if ((rc = ensure(c, &S.adv[d], FACEX + d, sizeof(T)))) return rc;
         }
     }
     T* psi = xant;
-    T* nxt = (T*)c->psi2;
-    T* adv[3] = {(T*)c->adv[0], (T*)c->adv[1], (T*)c->adv[2]};
-    T* lim[3] = {(T*)c->ul[0], (T*)c->ul[1], (T*)c->ul[2]};
+    T* nxt = (T*)S.psi2;
+    T* adv[3] = {(T*)S.adv[0], (T*)S.adv[1], (T*)S.adv[2]};
+    T* lim[3] = {(T*)S.ul[0], (T*)S.ul[1], (T*)S.ul[2]};
     for (int it = 2; it <= iters; it++) {
         const bool last = it == iters;
         const T* a1 = it == 2 ? u1 : adv[0];
@@ -285,6 +275,7 @@ template <typename T>
 int step_impl(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T* u1, const T* u2,
               const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
               hipStream_t s) {
+    Scratch& S = scr<T>(c);
     int rc;
     bool fused = false;
     if ((rc = check_step_opts(variant, iters, nonos))) return rc;
@@ -292,8 +283,8 @@ int step_impl(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T* u
     if ((rc = use_fused<T>(c, variant, iters, nonos, &fused))) return rc;
     if (!fused)
         return step_stages<T>(c, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy, s);
-    if ((rc = ensure(c, &c->xnext, SCAL))) return rc;
-    T* xn = (T*)c->xnext;
+    if ((rc = ensure(c, &S.xnext, SCAL, sizeof(T)))) return rc;
+    T* xn = (T*)S.xnext;
     HIPCHK(Fused<T>::step(c->g, x, xn, u1, u2, u3, rhr, h, bcx, bcy, s));
     HIPCHK(hipMemcpyAsync(x, xn, (size_t)len_of<SCAL>(c->g) * sizeof(T), hipMemcpyDeviceToDevice, s));
     return MPDATA_OK;
@@ -327,6 +318,7 @@ template <typename T>
 int timeloop_impl(mpdata_ctx* c, int variant, int iters, int nonos, int nt, T* x, const T* u1,
                   const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
                   const T* bcy, hipStream_t s) {
+    Scratch& S = scr<T>(c);
     int rc;
     bool fused = false;
     if (nt < 0) return fail(MPDATA_EINVAL, "nt %d", nt);
@@ -339,8 +331,8 @@ int timeloop_impl(mpdata_ctx* c, int variant, int iters, int nonos, int nt, T* x
                 return rc;
         return MPDATA_OK;
     }
-    if ((rc = ensure(c, &c->xnext, SCAL))) return rc;
-    T* buf[2] = {x, (T*)c->xnext};
+    if ((rc = ensure(c, &S.xnext, SCAL, sizeof(T)))) return rc;
+    T* buf[2] = {x, (T*)S.xnext};
     for (int it = 0; it < nt; it++)
         HIPCHK(Fused<T>::step(c->g, buf[it & 1], buf[(it + 1) & 1], u1, u2, u3, rhr, h, bcx, bcy, s));
     if (nt & 1)
@@ -629,6 +621,88 @@ int upwind_c_timeloop(void* xtracer, void* uadv, void* vadv, void* wadv, void* r
     if (bc) (void)hipFree(bc);
     mpdata_ctx_destroy(c);
     return rc;
+}
+
+// ---- tangent-linear step ----------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+using D64 = Dual<double>;
+
+__global__ void k_pack_dual(const double* __restrict__ v, const double* __restrict__ d,
+                            D64* __restrict__ out, long long n) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+         q += (long long)gridDim.x * blockDim.x)
+        out[q] = D64(v[q], d ? d[q] : 0.0);
+}
+
+__global__ void k_unpack_dual(const D64* __restrict__ in, double* __restrict__ v,
+                              double* __restrict__ d, long long n) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
+         q += (long long)gridDim.x * blockDim.x) {
+        const D64 a = in[q];
+        v[q] = a.v;
+        d[q] = a.d;
+    }
+}
+
+int grid_of(long long n) { return (int)std::min<long long>(2048, (n + 255) / 256); }
+
+int tl_step_impl(mpdata_ctx* c, int variant, int iters, int nonos, double* x, double* dx,
+                 const double* u1, const double* u2, const double* u3, const double* du1,
+                 const double* du2, const double* du3, const double* rhr, const double* h,
+                 const double* bcx, const double* bcy, hipStream_t s) {
+    int rc;
+    if ((rc = check_step_opts(variant, iters, nonos))) return rc;
+    if ((rc = check_step_args<double>(c, x, u1, u2, u3, rhr, h, bcx, bcy))) return rc;
+    if (!dx) return fail(MPDATA_EINVAL, "null tangent dx");
+    Scratch& S = c->dual;
+    const Geo& g = c->g;
+    const int kinds[6] = {SCAL, FACEX, FACEY, FACEZ, SCAL, SCAL};
+    for (int q = 0; q < 6; q++)
+        if ((rc = ensure(c, &S.in[q], kinds[q], sizeof(D64)))) return rc;
+    // inflow values: (2, lp, mp) and (2, lp, np) arrays, sized as x faces to fit
+    if ((rc = ensure(c, &S.in[6], FACEX, sizeof(D64))) || (rc = ensure(c, &S.in[7], FACEX, sizeof(D64))))
+        return rc;
+    const double* vals[6] = {x, u1, u2, u3, rhr, h};
+    const double* tans[6] = {dx, du1, du2, du3, nullptr, nullptr};
+    for (int q = 0; q < 6; q++) {
+        const long long n = field_len(g, kinds[q]);
+        k_pack_dual<<<grid_of(n), 256, 0, s>>>(vals[q], tans[q], (D64*)S.in[q], n);
+    }
+    if (bcx) {
+        const long long n = 2ll * g.mp * g.lp;
+        k_pack_dual<<<grid_of(n), 256, 0, s>>>(bcx, nullptr, (D64*)S.in[6], n);
+    }
+    if (bcy) {
+        const long long n = 2ll * g.np * g.lp;
+        k_pack_dual<<<grid_of(n), 256, 0, s>>>(bcy, nullptr, (D64*)S.in[7], n);
+    }
+    HIPCHK(hipGetLastError());
+    D64* X = (D64*)S.in[0];
+    rc = step_stages<D64>(c, variant, iters, nonos, X, (const D64*)S.in[1], (const D64*)S.in[2],
+                          (const D64*)S.in[3], (const D64*)S.in[4], (const D64*)S.in[5],
+                          bcx ? (const D64*)S.in[6] : nullptr, bcy ? (const D64*)S.in[7] : nullptr, s);
+    if (rc) return rc;
+    const long long n = field_len(g, SCAL);
+    k_unpack_dual<<<grid_of(n), 256, 0, s>>>(X, x, dx, n);
+    HIPCHK(hipGetLastError());
+    return MPDATA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpdata_tl_step_f64(mpdata_ctx* ctx, int variant, int iters, int nonos, double* x, double* dx,
+                       const double* u1, const double* u2, const double* u3, const double* du1,
+                       const double* du2, const double* du3, const double* rhr, const double* h,
+                       const double* bcx, const double* bcy, void* stream) {
+    int rc = check_ctx(ctx, 8);
+    if (rc) return rc;
+    return tl_step_impl(ctx, variant, iters, nonos, x, dx, u1, u2, u3, du1, du2, du3, rhr, h, bcx,
+                        bcy, (hipStream_t)stream);
 }
 
 int mpdata_dev_alloc(long long bytes, void** ptr) {

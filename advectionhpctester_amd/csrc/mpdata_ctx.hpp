@@ -1,0 +1,63 @@
+// The per-domain context behind the C ABI's opaque mpdata_ctx: geometry,
+// execution path and device scratch (allocated on first use, zero-filled
+// once, owned until mpdata_ctx_destroy).
+#pragma once
+#include <vector>
+
+#include "mpdata_common.hpp"
+
+namespace mpdata {
+
+// One set of scratch fields; element size fixed by the set's use.
+struct Scratch {
+    void* xant = nullptr;
+    void* psi2 = nullptr;
+    void* xnext = nullptr;  // output buffer of the out-of-place fused step
+    void* v[3] = {nullptr, nullptr, nullptr};
+    void* cp = nullptr;
+    void* cn = nullptr;
+    void* fns[3] = {nullptr, nullptr, nullptr};
+    void* f[3] = {nullptr, nullptr, nullptr};
+    void* ul[3] = {nullptr, nullptr, nullptr};
+    void* adv[3] = {nullptr, nullptr, nullptr};
+    // packed inputs of the tangent-linear step (value, tangent pairs)
+    void* in[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+// Forward intermediates of one recorded step (adjoint tape slot).
+struct TapeSlot {
+    void* xb = nullptr;     // x with the inflow halo (upwind input)
+    void* x0 = nullptr;     // x with the edge-copied halo (limiter bounds)
+    void* psi = nullptr;    // upwind result with the mirrored halo
+    void* v[3] = {nullptr, nullptr, nullptr};
+    void* fns[3] = {nullptr, nullptr, nullptr};
+    void* vl[3] = {nullptr, nullptr, nullptr};
+    void* cp = nullptr;
+    void* cn = nullptr;
+};
+
+}  // namespace mpdata
+
+struct mpdata_ctx {
+    mpdata::Geo g;
+    int real_bytes;
+    int path;
+    int device;
+    mpdata::Scratch prim;   // scratch in the context's real type
+    mpdata::Scratch dual;   // scratch of the tangent-linear step
+    void* red = nullptr;    // reduction partials (device)
+    long long red_bytes = 0;
+    std::vector<mpdata::TapeSlot> tape;
+    // adjoint work arrays
+    void* adj[24] = {};
+    long long held = 0;     // bytes of device scratch
+    std::vector<void*> owned;
+};
+
+namespace mpdata {
+
+long long field_len(const Geo& g, int kind);
+// Allocates *slot (field of `kind`, elem_bytes per element) unless set.
+int ensure(mpdata_ctx* c, void** slot, int kind, int elem_bytes);
+
+}  // namespace mpdata

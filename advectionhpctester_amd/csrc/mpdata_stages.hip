@@ -9,6 +9,7 @@
 // 64-wide wavefront per row segment), k on grid.z; the ground/sky plane
 // special cases of the reference are branches on k that are uniform per
 // block.
+#include "mpdata_dual.hpp"
 #include "mpdata_stages.hpp"
 
 namespace mpdata {
@@ -795,5 +796,28 @@ hipError_t Stages<T>::norm_sums(const Geo& g, const T* x, const T* x0, double* p
 
 template struct Stages<double>;
 template struct Stages<float>;
+
+// The stages of one step on (value, tangent) pairs: the tangent-linear step.
+using D64 = Dual<double>;
+template hipError_t Stages<D64>::putbcinxh(const Geo&, D64*, const D64*, const D64*, hipStream_t);
+template hipError_t Stages<D64>::halo_xy(const Geo&, D64*, int, int, hipStream_t);
+template hipError_t Stages<D64>::face_bc(const Geo&, D64*, D64*, D64*, hipStream_t);
+template hipError_t Stages<D64>::upwind(const Geo&, const D64*, const D64*, const D64*,
+                                        const D64*, D64*, const D64*, const D64*, hipStream_t);
+template hipError_t Stages<D64>::pseudo_vel(const Geo&, int, const D64*, const D64*, const D64*,
+                                            const D64*, const D64*, D64*, D64*, D64*, D64*, D64*,
+                                            D64*, hipStream_t);
+template hipError_t Stages<D64>::limiter(const Geo&, const D64*, const D64*, const D64*,
+                                         const D64*, const D64*, const D64*, D64*, D64*,
+                                         hipStream_t);
+template hipError_t Stages<D64>::gauge_update(const Geo&, int, const D64*, const D64*, const D64*,
+                                              const D64*, const D64*, const D64*, const D64*,
+                                              const D64*, D64*, D64*, D64*, D64*, hipStream_t);
+template hipError_t Stages<D64>::std_flux(const Geo&, int, const D64*, const D64*, const D64*,
+                                          const D64*, const D64*, const D64*, const D64*, D64*,
+                                          D64*, D64*, D64*, D64*, D64*, hipStream_t);
+template hipError_t Stages<D64>::std_update(const Geo&, const D64*, const D64*, const D64*,
+                                            const D64*, const D64*, const D64*, D64*, hipStream_t);
+template hipError_t Stages<D64>::copy_interior(const Geo&, const D64*, D64*, hipStream_t);
 
 }  // namespace mpdata

@@ -195,6 +195,42 @@ class Domain:
                                             self._p(x, 0, "x"), pu1, pu2, pu3, prhr, ph, pbx, pby,
                                             self._stream()))
 
+    # -- tangent-linear and adjoint steps ----------------------------------------
+    def tl_step(self, x, dx, u1, u2, u3, du1, du2, du3, rhr, h, bcx=None, bcy=None,
+                variant=GAUGE, iters=2, nonos=1):
+        """x <- step(x) and dx <- its derivative along (dx, du1, du2, du3), in
+        place (FP64; du may be None for zero)."""
+        if self.dtype != torch.float64:
+            raise MpdataError(_lib.MPDATA_EINVAL, "the tangent-linear step is FP64")
+        pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
+        pbx, pby = self._bc(bcx, bcy)
+        pd = [self._p(t, k, n, allow_none=True) for t, k, n in
+              ((du1, 1, "du1"), (du2, 2, "du2"), (du3, 3, "du3"))]
+        check(LIB.mpdata_tl_step_f64(self._ctx, variant, iters, nonos, self._p(x, 0, "x"),
+                                     self._p(dx, 0, "dx"), pu1, pu2, pu3, *pd, prhr, ph, pbx, pby,
+                                     self._stream()))
+
+    def adj_reserve(self, nslots):
+        check(LIB.mpdata_adj_reserve(self._ctx, int(nslots)))
+
+    def adj_record(self, slot, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy):
+        """One standard non-oscillatory step x_in -> x_out, keeping its
+        intermediates in tape slot `slot`."""
+        pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
+        pbx, pby = self._bc(bcx, bcy)
+        check(LIB.mpdata_adj_record_f64(self._ctx, int(slot), self._p(x_in, 0, "x_in"),
+                                        self._p(x_out, 0, "x_out"), pu1, pu2, pu3, prhr, ph,
+                                        pbx, pby, self._stream()))
+
+    def adj_step(self, slot, u1, u2, u3, rhr, h, xbar, ubar1, ubar2, ubar3):
+        """xbar (adjoint of the recorded step's output) <- adjoint of its
+        input, in place; ubar1..3 += the velocity adjoint."""
+        pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
+        check(LIB.mpdata_adj_step_f64(self._ctx, int(slot), pu1, pu2, pu3, prhr, ph,
+                                      self._p(xbar, 0, "xbar"), self._p(ubar1, 1, "ubar1"),
+                                      self._p(ubar2, 2, "ubar2"), self._p(ubar3, 3, "ubar3"),
+                                      self._stream()))
+
     # -- velocity predictor ----------------------------------------------------
     def _levels(self, t, name):
         return self._p(t, (3,) + self.shape(0), name)

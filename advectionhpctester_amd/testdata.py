@@ -412,3 +412,74 @@ def ensemble_member(np_, mp, lp, seed, ih=1, dtype=None, device="cpu"):
     return dict(x=x, u1=u1, u2=u2, u3=u3, h=h, rhr=h.clone(),
                 bcx=torch.zeros((2, lp, mp), dtype=dtype, device=device),
                 bcy=torch.zeros((2, lp, np_), dtype=dtype, device=device))
+
+
+# ---------------------------------------------------------------------------
+# Gradient test case: standard MPDATA under the deformational flow
+# ---------------------------------------------------------------------------
+def adjoint_case(np_=128, mp=128, lp=64, nsteps=50, period=200, seed=5, ih=1):
+    """Initial field: a Gaussian exp(-r^2/R^2), R = np/8 cells, at 0.4 of
+    each axis, on a smooth background 0.1 + 0.05 sin sin cos (long waves, so
+    that no two neighbouring values tie and the limiter's max/min selections
+    are locally unique); observation: the same Gaussian moved by
+    (3, -2, 1) cells with seeded uniform noise in [-0.01, 0.01]; face
+    velocities: the deformational flow (deformational_velocity, max Courant
+    0.7) by A-to-C averaging, scaled at step n by cos(pi (n + 1/2) / period)
+    (the returned `factors`); h = rhr = 1, zero inflow.  numpy, FP64."""
+    rng = np.random.default_rng(seed)
+    R = np_ / 8.0
+
+    def gauss(dx, dy, dz):
+        i = np.arange(1, np_ + 1)[None, None, :]
+        j = np.arange(1, mp + 1)[None, :, None]
+        k = np.arange(1, lp + 1)[:, None, None]
+        r2 = ((i - 0.4 * np_ - dx) ** 2 + (j - 0.4 * mp - dy) ** 2 + (k - 0.4 * lp - dz) ** 2)
+        bg = 0.1 + 0.05 * (np.sin(2 * np.pi * (1.3 * (i - 0.5) / np_ + 0.1)) *
+                           np.sin(2 * np.pi * (0.9 * (j - 0.5) / mp + 0.2)) *
+                           np.cos(2 * np.pi * 0.7 * (k - 0.5) / lp))
+        f = np.zeros(shape(np_, mp, lp, ih))
+        f[ih:ih + lp, ih:ih + mp, ih:ih + np_] = bg + np.exp(-r2 / R ** 2)
+        return f
+
+    x = gauss(0, 0, 0)
+    obs = gauss(3, -2, 1)
+    obs[ih:ih + lp, ih:ih + mp, ih:ih + np_] += rng.uniform(-0.01, 0.01, (lp, mp, np_))
+    ua, va, wa = deformational_velocity(np_, mp, lp, cmax=0.7, ih=ih)
+    u1, u2, u3 = a_to_c_open(ua, va, wa, ih)
+    h = np.ones(shape(np_, mp, lp, ih))
+    factors = [math.cos(math.pi * (n + 0.5) / period) for n in range(nsteps)]
+    return dict(x=x, obs=obs, u1=u1, u2=u2, u3=u3, h=h, rhr=h.copy(),
+                bcx=np.zeros((2, lp, mp)), bcy=np.zeros((2, lp, np_)), factors=factors)
+
+
+def switch_free_case(np_=24, mp=20, lp=12, ih=1):
+    """A case whose standard non-oscillatory step has no branch switch near
+    the data: a field increasing along each axis by 0.010, 0.008 and 0.012
+    per cell (distinct, so the limiter's max/min selections have margins of
+    0.002) with a small smooth ripple, face velocities positive everywhere
+    (0.3, 0.2, 0.1 with smooth ripples), inflow values that continue the
+    field, h = rhr = 1.  With these gradients and velocities each
+    antidiffusive velocity keeps the sign of its main term,
+    (1 - u1) gx > u2 gy + u3 gz and likewise in y and z, and the limiter
+    ratios stay well above 1 away from the outflow corners, so central
+    differences converge to the derivative.  numpy, FP64."""
+    def field(i, j, k):
+        return (1.0 + 0.010 * i + 0.008 * j + 0.012 * k +
+                0.001 * np.sin(2 * np.pi * i / np_) * np.cos(2 * np.pi * j / mp))
+
+    k, j, i = np.meshgrid(*(np.arange(n) for n in shape(np_, mp, lp, ih)), indexing="ij")
+    x = field(i, j, k).astype(np.float64)
+
+    def vel(kind, base, amp):
+        s = shape(np_, mp, lp, ih, kind)
+        kk, jj, ii = np.meshgrid(*(np.arange(n) for n in s), indexing="ij")
+        return base + amp * np.sin(2 * np.pi * (ii / np_ + 0.3)) * np.cos(2 * np.pi * jj / mp) * \
+            np.cos(np.pi * kk / lp)
+
+    u1, u2, u3 = vel(1, 0.3, 0.01), vel(2, 0.2, 0.01), vel(3, 0.1, 0.005)
+    kk, jj = np.meshgrid(np.arange(1, lp + 1), np.arange(1, mp + 1), indexing="ij")
+    bcx = np.stack([field(0, jj, kk), field(np_ + 1, jj, kk)])
+    kk, ii = np.meshgrid(np.arange(1, lp + 1), np.arange(1, np_ + 1), indexing="ij")
+    bcy = np.stack([field(ii, 0, kk), field(ii, mp + 1, kk)])
+    h = np.ones(shape(np_, mp, lp, ih))
+    return dict(x=x, u1=u1, u2=u2, u3=u3, h=h, rhr=h.copy(), bcx=bcx, bcy=bcy)

@@ -241,6 +241,32 @@ int mpdata_norm_sums_f64(mpdata_ctx *ctx, const double *x, const double *x0, dou
 int mpdata_norm_sums_f32(mpdata_ctx *ctx, const float *x, const float *x0, double *out,
                          void *stream);
 
+/* ---- tangent-linear and adjoint steps --------------------------------------
+ * No counterpart in the reference.  The tangent-linear step advances x and
+ * its tangent dx (directional derivative along dx, du1..du3; NULL du = 0)
+ * through mpdata_step's stage sequence on (value, tangent) pairs, for every
+ * variant/iters/nonos the per-stage path supports.  Synchronous w.r.t. the
+ * stream only. */
+int mpdata_tl_step_f64(mpdata_ctx *ctx, int variant, int iters, int nonos, double *x, double *dx,
+                       const double *u1, const double *u2, const double *u3, const double *du1,
+                       const double *du2, const double *du3, const double *rhr, const double *h,
+                       const double *bcx, const double *bcy, void *stream);
+/* Adjoint of the standard non-oscillatory 2-iteration step (open walls,
+ * ih = 1, FP64 context).  mpdata_adj_reserve allocates nslots tape slots;
+ * mpdata_adj_record_f64 runs one forward step x_in -> x_out (x_out as
+ * mpdata_step leaves x) keeping its intermediates in `slot`;
+ * mpdata_adj_step_f64 maps xbar (adjoint of the interior of x_out) to the
+ * adjoint of the interior of x_in, in place, and ADDS the velocity adjoint
+ * to ubar1..ubar3 (zero them before the first step). */
+int mpdata_adj_reserve(mpdata_ctx *ctx, int nslots);
+int mpdata_adj_record_f64(mpdata_ctx *ctx, int slot, const double *x_in, double *x_out,
+                          const double *u1, const double *u2, const double *u3,
+                          const double *rhr, const double *h, const double *bcx,
+                          const double *bcy, void *stream);
+int mpdata_adj_step_f64(mpdata_ctx *ctx, int slot, const double *u1, const double *u2,
+                        const double *u3, const double *rhr, const double *h, double *xbar,
+                        double *ubar1, double *ubar2, double *ubar3, void *stream);
+
 /* ---- ensembles: many domains in one launch ---------------------------------
  * No counterpart in the reference (its batching is many fields on one grid,
  * advection_CEtimestep_twostep_GPUBC.f90).  n members of different shapes
