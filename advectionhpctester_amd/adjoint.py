@@ -52,8 +52,7 @@ class MpdataSteps(torch.autograd.Function):
         xbar = torch.zeros_like(grad)
         mpdata.interior(xbar).copy_(mpdata.interior(grad))
         ub = [torch.zeros_like(u) for u in (u1, u2, u3)]
-        ubn = [torch.empty_like(u) for u in (u1, u2, u3)]
-        bufs = [torch.empty_like(grad), torch.empty_like(grad)]
+        bufs = [torch.zeros_like(grad), torch.zeros_like(grad)]
         for seg in reversed(range(len(ctx.ckpts))):
             s0 = seg * ckpt
             s1 = min(nsteps, s0 + ckpt)
@@ -66,15 +65,8 @@ class MpdataSteps(torch.autograd.Function):
                 dom.adj_record(n - s0, x, out, *un, rhr, h, bcx, bcy)
                 x = out
             for n in reversed(range(s0, s1)):
-                f = factors[n]
-                if f == 1.0:
-                    dom.adj_step(n - s0, *vel[n - s0], rhr, h, xbar, *ub)
-                else:
-                    for t in ubn:
-                        t.zero_()
-                    dom.adj_step(n - s0, *vel[n - s0], rhr, h, xbar, *ubn)
-                    for a, b in zip(ub, ubn):
-                        a.add_(b, alpha=f)
+                # u_n = f_n u: the gradient of u collects f_n times that of u_n
+                dom.adj_step(n - s0, *vel[n - s0], rhr, h, xbar, *ub, ufac=factors[n])
         ctx.ckpts = None
         return (xbar, ub[0], ub[1], ub[2]) + (None,) * 8
 

@@ -53,12 +53,45 @@ const int A_KIND[A_N] = {SCAL, SCAL, SCAL, SCAL, SCAL, FACEX, FACEY, FACEZ, FACE
 
 struct Adj {
     R *psi, *x0, *cp, *cn, *sum, *v1, *v2, *v3, *f1, *f2, *f3;
+    R uf;  // the velocity adjoint is added to ubar scaled by uf
 };
 
-__global__ void k_zero(R* __restrict__ a, long long n) {
+// Zeroes up to 8 arrays in one launch (blockIdx.y = array).
+struct ZeroList {
+    R* p[8];
+    long long n[8];
+};
+__global__ void k_zero(ZeroList z) {
+    R* __restrict__ a = z.p[blockIdx.y];
+    const long long n = z.n[blockIdx.y];
     for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
          q += (long long)gridDim.x * blockDim.x)
         a[q] = R(0);
+}
+
+// The two x inputs of a recorded step from x_in in one pass: xb with the
+// inflow halo (putbcinxh, mpdataoperators.inc:253-287) for the upwind, x0
+// with the edge-copied halo (cp_last_to_halo_xy, :551-584) for the limiter;
+// all other cells are x_in.
+__global__ void k_prep_x(Geo g, const R* __restrict__ x, const R* __restrict__ bcx,
+                         const R* __restrict__ bcy, R* __restrict__ xb, R* __restrict__ x0) {
+    const int ih = g.ih;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 1 - ih;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 1 - ih;
+    const int k = blockIdx.z + 1 - ih;
+    if (i > g.np + ih || j > g.mp + ih) return;
+    const long long c = off_raw<SCAL>(g, i, j, k);
+    const bool kin = k >= 1 && k <= g.lp, iin = i >= 1 && i <= g.np, jin = j >= 1 && j <= g.mp;
+    R vb = x[c], v0 = vb;
+    if (kin && jin && (i == 0 || i == g.np + 1)) {
+        vb = bcx[(j - 1) + (long long)g.mp * ((k - 1) + (long long)g.lp * (i == 0 ? 0 : 1))];
+        v0 = x[off_raw<SCAL>(g, i == 0 ? 1 : g.np, j, k)];
+    } else if (kin && iin && (j == 0 || j == g.mp + 1)) {
+        vb = bcy[(i - 1) + (long long)g.np * ((k - 1) + (long long)g.lp * (j == 0 ? 0 : 1))];
+        v0 = x[off_raw<SCAL>(g, i, j == 0 ? 1 : g.mp, k)];
+    }
+    xb[c] = vb;
+    x0[c] = v0;
 }
 
 // ---- update: out = rhr*(psi/rhr - sum*hinv) (antidiff3d-standard-gpubc.f90:431-459)
@@ -73,19 +106,46 @@ __global__ void __launch_bounds__(256) k_adj_update(Geo g, const R* __restrict__
     S_(a.sum, i, j, k) = -(w * hinv);
 }
 
+// ---- block-local accumulation ---------------------------------------------
+// Adjoint kernels scatter into neighbouring cells and faces.  A block of
+// 64x4 cells (one k plane) sums its contributions in LDS regions that cover
+// every target of its threads and then adds each region entry to the global
+// array with one atomic: far fewer global atomics, and no contention inside
+// the block.
+constexpr int BX = 64, BY = 4;
+
+struct Region {   // LDS slot of (x, y, z) = off + ((z - z0) * ny + (y - y0)) * nx + (x - x0)
+    int x0, y0, z0, nx, ny, nz, off;
+    __device__ int at(int x, int y, int z) const { return off + ((z - z0) * ny + (y - y0)) * nx + (x - x0); }
+    __device__ int size() const { return nx * ny * nz; }
+};
+
+template <int KIND>
+__device__ void flush_region(const Geo& g, const R* L, const Region& r, R* a, int tid) {
+    const int ih = g.ih;
+    const int xmax = g.np + ih + (KIND == FACEX), ymax = g.mp + ih + (KIND == FACEY);
+    const int zmax = g.lp + ih + (KIND == FACEZ);
+    for (int q = tid; q < r.size(); q += BX * BY) {
+        const R v = L[r.off + q];
+        if (v == R(0)) continue;
+        const int x = r.x0 + q % r.nx, y = r.y0 + (q / r.nx) % r.ny, z = r.z0 + q / (r.nx * r.ny);
+        if (x <= xmax && y <= ymax && z <= zmax) atomicAdd(&a[off_raw<KIND>(g, x, y, z)], v);
+    }
+}
+
 // ---- limited donor fluxes (:372-407), one face ----------------------------
 // fb = adjoint of the flux; m/c = upstream/downstream cells
-__device__ __forceinline__ void adj_face_flux(R fb, R v, R vl, R pm, R pc, R rm, R rc,
-                                              R cp_c, R cn_m, R cp_m, R cn_c, R* psib_m,
-                                              R* psib_c, R* cpb_c, R* cnb_m, R* cpb_m,
-                                              R* cnb_c, R* vbar) {
+__device__ __forceinline__ void adj_face_flux(R* L, R fb, R v, R vl, R pm, R pc, R rm, R rc,
+                                              R cp_c, R cn_m, R cp_m, R cn_c, int psib_m,
+                                              int psib_c, int cpb_c, int cnb_m, int cpb_m,
+                                              int cnb_c, R* vbar) {
     const R y1 = pm / rm, y2 = pc / rc;
     // f = donor(y1, y2, vl)
     const R y1b = fb * (vl > R(0) ? vl : R(0));
     const R y2b = fb * (vl < R(0) ? vl : R(0));
     const R vlb = fb * (ind_pos(vl) * y1 + ind_neg(vl) * y2);
-    acc(psib_m, y1b / rm);
-    acc(psib_c, y2b / rc);
+    atomicAdd(&L[psib_m], y1b / rm);
+    atomicAdd(&L[psib_c], y2b / rc);
     // vl = slimv(v, a1, a2, pm, pc)
     const R Pm = copysign(R(1), pm) > R(0) ? R(1) : R(0);
     const R Nm = copysign(R(1), -pm) > R(0) ? R(1) : R(0);
@@ -102,57 +162,73 @@ __device__ __forceinline__ void adj_face_flux(R fb, R v, R vl, R pm, R pc, R rm,
     const R a2b = vlb * (ppv * Nm - pnv * Pc);
     // a1 = min3(1, cp_c, cn_m), a2 = min3(1, cp_m, cn_c)
     if (m1 < cn_m) {
-        if (!(R(1) < cp_c)) acc(cpb_c, a1b);
+        if (!(R(1) < cp_c)) atomicAdd(&L[cpb_c], a1b);
     } else {
-        acc(cnb_m, a1b);
+        atomicAdd(&L[cnb_m], a1b);
     }
     if (m2 < cn_c) {
-        if (!(R(1) < cp_m)) acc(cpb_m, a2b);
+        if (!(R(1) < cp_m)) atomicAdd(&L[cpb_m], a2b);
     } else {
-        acc(cnb_c, a2b);
+        atomicAdd(&L[cnb_c], a2b);
     }
 }
 
-__global__ void __launch_bounds__(256) k_adj_flux(Geo g, const R* __restrict__ v1,
-                                                  const R* __restrict__ v2,
-                                                  const R* __restrict__ v3,
-                                                  const R* __restrict__ vl1,
-                                                  const R* __restrict__ vl2,
-                                                  const R* __restrict__ vl3,
-                                                  const R* __restrict__ cp,
-                                                  const R* __restrict__ cn,
-                                                  const R* __restrict__ psi,
-                                                  const R* __restrict__ rhr, Adj a) {
-    CELL_HEAD
-    const R pc = S_(psi, i, j, k), rc = S_(rhr, i, j, k);
-    const R sc = S_(a.sum, i, j, k);
-    if (i > 1) {
-        const R fb = S_(a.sum, i - 1, j, k) - sc;
-        adj_face_flux(fb, X_(v1, i, j, k), X_(vl1, i, j, k), S_(psi, i - 1, j, k), pc,
-                      S_(rhr, i - 1, j, k), rc, S_(cp, i, j, k), S_(cn, i - 1, j, k),
-                      S_(cp, i - 1, j, k), S_(cn, i, j, k), SP(a.psi, i - 1, j, k),
-                      SP(a.psi, i, j, k), SP(a.cp, i, j, k), SP(a.cn, i - 1, j, k),
-                      SP(a.cp, i - 1, j, k), SP(a.cn, i, j, k), XP(a.v1, i, j, k));
+__global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict__ v1,
+                                                      const R* __restrict__ v2,
+                                                      const R* __restrict__ v3,
+                                                      const R* __restrict__ vl1,
+                                                      const R* __restrict__ vl2,
+                                                      const R* __restrict__ vl3,
+                                                      const R* __restrict__ cp,
+                                                      const R* __restrict__ cn,
+                                                      const R* __restrict__ psi,
+                                                      const R* __restrict__ rhr, Adj a) {
+    // targets: the cell and its three low neighbours
+    constexpr int N = (BX + 1) * (BY + 1) * 2;
+    __shared__ R L[3 * N];
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const int i0 = 1 + blockIdx.x * BX, j0 = 1 + blockIdx.y * BY, k = 1 + blockIdx.z;
+    const Region rp{i0 - 1, j0 - 1, k - 1, BX + 1, BY + 1, 2, 0};
+    const Region rcp{i0 - 1, j0 - 1, k - 1, BX + 1, BY + 1, 2, N};
+    const Region rcn{i0 - 1, j0 - 1, k - 1, BX + 1, BY + 1, 2, 2 * N};
+    for (int q = tid; q < 3 * N; q += BX * BY) L[q] = R(0);
+    __syncthreads();
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    if (i <= g.np && j <= g.mp) {
+        const bool sky = k == g.lp;
+        const R pc = S_(psi, i, j, k), rc = S_(rhr, i, j, k);
+        const R sc = S_(a.sum, i, j, k);
+        if (i > 1) {
+            const R fb = S_(a.sum, i - 1, j, k) - sc;
+            adj_face_flux(L, fb, X_(v1, i, j, k), X_(vl1, i, j, k), S_(psi, i - 1, j, k), pc,
+                          S_(rhr, i - 1, j, k), rc, S_(cp, i, j, k), S_(cn, i - 1, j, k),
+                          S_(cp, i - 1, j, k), S_(cn, i, j, k), rp.at(i - 1, j, k), rp.at(i, j, k),
+                          rcp.at(i, j, k), rcn.at(i - 1, j, k), rcp.at(i - 1, j, k),
+                          rcn.at(i, j, k), XP(a.v1, i, j, k));
+        }
+        if (j > 1) {
+            const R fb = S_(a.sum, i, j - 1, k) - sc;
+            adj_face_flux(L, fb, Y_(v2, i, j, k), Y_(vl2, i, j, k), S_(psi, i, j - 1, k), pc,
+                          S_(rhr, i, j - 1, k), rc, S_(cp, i, j, k), S_(cn, i, j - 1, k),
+                          S_(cp, i, j - 1, k), S_(cn, i, j, k), rp.at(i, j - 1, k), rp.at(i, j, k),
+                          rcp.at(i, j, k), rcn.at(i, j - 1, k), rcp.at(i, j - 1, k),
+                          rcn.at(i, j, k), YP(a.v2, i, j, k));
+        }
+        if (k > 1) {
+            // the ground cell counts its upper face twice, the sky cell its lower
+            const R c1 = k - 1 == 1 ? R(2) : R(1), c2 = sky ? R(2) : R(1);
+            const R fb = c1 * S_(a.sum, i, j, k - 1) - c2 * sc;
+            adj_face_flux(L, fb, Z_(v3, i, j, k), Z_(vl3, i, j, k), S_(psi, i, j, k - 1), pc,
+                          S_(rhr, i, j, k - 1), rc, S_(cp, i, j, k), S_(cn, i, j, k - 1),
+                          S_(cp, i, j, k - 1), S_(cn, i, j, k), rp.at(i, j, k - 1), rp.at(i, j, k),
+                          rcp.at(i, j, k), rcn.at(i, j, k - 1), rcp.at(i, j, k - 1),
+                          rcn.at(i, j, k), ZP(a.v3, i, j, k));
+        }
     }
-    if (j > 1) {
-        const R fb = S_(a.sum, i, j - 1, k) - sc;
-        adj_face_flux(fb, Y_(v2, i, j, k), Y_(vl2, i, j, k), S_(psi, i, j - 1, k), pc,
-                      S_(rhr, i, j - 1, k), rc, S_(cp, i, j, k), S_(cn, i, j - 1, k),
-                      S_(cp, i, j - 1, k), S_(cn, i, j, k), SP(a.psi, i, j - 1, k),
-                      SP(a.psi, i, j, k), SP(a.cp, i, j, k), SP(a.cn, i, j - 1, k),
-                      SP(a.cp, i, j - 1, k), SP(a.cn, i, j, k), YP(a.v2, i, j, k));
-    }
-    if (k > 1) {
-        // the ground cell counts its upper face twice, the sky cell its lower
-        const R c1 = k - 1 == 1 ? R(2) : R(1), c2 = sky ? R(2) : R(1);
-        const R fb = c1 * S_(a.sum, i, j, k - 1) - c2 * sc;
-        adj_face_flux(fb, Z_(v3, i, j, k), Z_(vl3, i, j, k), S_(psi, i, j, k - 1), pc,
-                      S_(rhr, i, j, k - 1), rc, S_(cp, i, j, k), S_(cn, i, j, k - 1),
-                      S_(cp, i, j, k - 1), S_(cn, i, j, k), SP(a.psi, i, j, k - 1),
-                      SP(a.psi, i, j, k), SP(a.cp, i, j, k), SP(a.cn, i, j, k - 1),
-                      SP(a.cp, i, j, k - 1), SP(a.cn, i, j, k), ZP(a.v3, i, j, k));
-    }
-    (void)gnd;
+    __syncthreads();
+    flush_region<SCAL>(g, L, rp, a.psi, tid);
+    flush_region<SCAL>(g, L, rcp, a.cp, tid);
+    flush_region<SCAL>(g, L, rcn, a.cn, tid);
 }
 
 // ---- limiter (:260-347) ----------------------------------------------------
@@ -169,107 +245,138 @@ __device__ __forceinline__ void pick_min(Pick& p, R v, int src, int di, int dj, 
     if (!(p.val < v)) p = Pick{v, src, di, dj, dk};   // min2(lo, t) = lo < t ? lo : t
 }
 
-__global__ void __launch_bounds__(256) k_adj_limiter(Geo g, const R* __restrict__ x0,
-                                                     const R* __restrict__ psi,
-                                                     const R* __restrict__ h,
-                                                     const R* __restrict__ w1,
-                                                     const R* __restrict__ w2,
-                                                     const R* __restrict__ w3,
-                                                     const R* __restrict__ cp,
-                                                     const R* __restrict__ cn, Adj a) {
-    CELL_HEAD
-    const R ep = Eps<R>::v;
-    Pick mxo{S_(x0, i - 1, j, k), 0, -1, 0, 0}, mno = mxo;
-    Pick mx{S_(psi, i - 1, j, k), 1, -1, 0, 0}, mn = mx;
+__global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restrict__ x0,
+                                                         const R* __restrict__ psi,
+                                                         const R* __restrict__ h,
+                                                         const R* __restrict__ w1,
+                                                         const R* __restrict__ w2,
+                                                         const R* __restrict__ w3,
+                                                         const R* __restrict__ cp,
+                                                         const R* __restrict__ cn, Adj a) {
+    // targets: psi and x0 within one cell in every direction, the six faces
+    constexpr int NC = (BX + 2) * (BY + 2) * 3;
+    constexpr int N1 = (BX + 1) * BY, N2 = BX * (BY + 1), N3 = BX * BY * 2;
+    __shared__ R L[2 * NC + N1 + N2 + N3];
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const int i0 = 1 + blockIdx.x * BX, j0 = 1 + blockIdx.y * BY, k = 1 + blockIdx.z;
+    const Region rp{i0 - 1, j0 - 1, k - 1, BX + 2, BY + 2, 3, 0};
+    const Region rx{i0 - 1, j0 - 1, k - 1, BX + 2, BY + 2, 3, NC};
+    const Region r1{i0, j0, k, BX + 1, BY, 1, 2 * NC};
+    const Region r2{i0, j0, k, BX, BY + 1, 1, 2 * NC + N1};
+    const Region r3{i0, j0, k, BX, BY, 2, 2 * NC + N1 + N2};
+    for (int q = tid; q < 2 * NC + N1 + N2 + N3; q += BX * BY) L[q] = R(0);
+    __syncthreads();
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    if (i <= g.np && j <= g.mp) {
+        const bool gnd = k == 1, sky = k == g.lp;
+        const R ep = Eps<R>::v;
+        Pick mxo{S_(x0, i - 1, j, k), 0, -1, 0, 0}, mno = mxo;
+        Pick mx{S_(psi, i - 1, j, k), 1, -1, 0, 0}, mn = mx;
 #define ACC(arr, src, lo, hi, di, dj, dk)                      \
     {                                                          \
         const R t_ = S_(arr, i + di, j + dj, k + dk);          \
         pick_min(lo, t_, src, di, dj, dk);                     \
         pick_max(hi, t_, src, di, dj, dk);                     \
     }
-    ACC(x0, 0, mno, mxo, 0, 0, 0) ACC(x0, 0, mno, mxo, 1, 0, 0)
-    ACC(x0, 0, mno, mxo, 0, -1, 0) ACC(x0, 0, mno, mxo, 0, 1, 0)
-    ACC(psi, 1, mn, mx, 0, 0, 0) ACC(psi, 1, mn, mx, 1, 0, 0)
-    ACC(psi, 1, mn, mx, 0, -1, 0) ACC(psi, 1, mn, mx, 0, 1, 0)
-    if (!gnd) { ACC(x0, 0, mno, mxo, 0, 0, -1) ACC(psi, 1, mn, mx, 0, 0, -1) }
-    if (!sky) { ACC(x0, 0, mno, mxo, 0, 0, 1) ACC(psi, 1, mn, mx, 0, 0, 1) }
+        ACC(x0, 0, mno, mxo, 0, 0, 0) ACC(x0, 0, mno, mxo, 1, 0, 0)
+        ACC(x0, 0, mno, mxo, 0, -1, 0) ACC(x0, 0, mno, mxo, 0, 1, 0)
+        ACC(psi, 1, mn, mx, 0, 0, 0) ACC(psi, 1, mn, mx, 1, 0, 0)
+        ACC(psi, 1, mn, mx, 0, -1, 0) ACC(psi, 1, mn, mx, 0, 1, 0)
+        if (!gnd) { ACC(x0, 0, mno, mxo, 0, 0, -1) ACC(psi, 1, mn, mx, 0, 0, -1) }
+        if (!sky) { ACC(x0, 0, mno, mxo, 0, 0, 1) ACC(psi, 1, mn, mx, 0, 0, 1) }
 #undef ACC
-    if (!(mx.val > mxo.val)) mx = mxo;   // mx = max2(mx, mxo)
-    if (!(mn.val < mno.val)) mn = mno;   // mn = min2(mn, mno)
-    const R a1p = X_(w1, i + 1, j, k), a1 = X_(w1, i, j, k);
-    const R a2p = Y_(w2, i, j + 1, k), a2 = Y_(w2, i, j, k);
-    R a3p, a3;
-    if (gnd) { a3p = Z_(w3, i, j, k + 1); a3 = -a3p; }
-    else if (sky) { a3 = Z_(w3, i, j, k); a3p = -a3; }
-    else { a3p = Z_(w3, i, j, k + 1); a3 = Z_(w3, i, j, k); }
-    const R hc = S_(h, i, j, k);
-    const R qp = (a1p < R(0) ? -a1p : R(0)) + (a1 > R(0) ? a1 : R(0)) + (a2p < R(0) ? -a2p : R(0)) +
-                 (a2 > R(0) ? a2 : R(0)) + (a3p < R(0) ? -a3p : R(0)) + (a3 > R(0) ? a3 : R(0)) + ep;
-    const R qn = (a1p > R(0) ? a1p : R(0)) + (a1 < R(0) ? -a1 : R(0)) + (a2p > R(0) ? a2p : R(0)) +
-                 (a2 < R(0) ? -a2 : R(0)) + (a3p > R(0) ? a3p : R(0)) + (a3 < R(0) ? -a3 : R(0)) + ep;
-    const R cpb = S_(a.cp, i, j, k), cnb = S_(a.cn, i, j, k);
-    const R tpb = cpb / qp, tnb = cnb / qn;
-    const R qpb = -(cpb * S_(cp, i, j, k) / qp);
-    const R qnb = -(cnb * S_(cn, i, j, k) / qn);
-    // cp = (mx - xc)*hc/qp, cn = (xc - mn)*hc/qn
-    const R mxb = tpb * hc, mnb = -(tnb * hc);
-    acc(SP(a.psi, i, j, k), -(tpb * hc) + tnb * hc);
-    acc(mx.src ? SP(a.psi, i + mx.di, j + mx.dj, k + mx.dk) : SP(a.x0, i + mx.di, j + mx.dj, k + mx.dk), mxb);
-    acc(mn.src ? SP(a.psi, i + mn.di, j + mn.dj, k + mn.dk) : SP(a.x0, i + mn.di, j + mn.dj, k + mn.dk), mnb);
-    // denominators: d pn(y) = -[y<0] dy, d pp(y) = [y>0] dy
-    const R b1p = -ind_neg(a1p) * qpb + ind_pos(a1p) * qnb;  // a1p enters qp as pn, qn as pp
-    const R b1 = ind_pos(a1) * qpb - ind_neg(a1) * qnb;
-    const R b2p = -ind_neg(a2p) * qpb + ind_pos(a2p) * qnb;
-    const R b2 = ind_pos(a2) * qpb - ind_neg(a2) * qnb;
-    const R b3p = -ind_neg(a3p) * qpb + ind_pos(a3p) * qnb;
-    const R b3 = ind_pos(a3) * qpb - ind_neg(a3) * qnb;
-    // wall faces of x and y are constants after the boundary condition
-    if (i < g.np) acc(XP(a.f1, i + 1, j, k), b1p);
-    if (i > 1) acc(XP(a.f1, i, j, k), b1);
-    if (j < g.mp) acc(YP(a.f2, i, j + 1, k), b2p);
-    if (j > 1) acc(YP(a.f2, i, j, k), b2);
-    if (gnd) acc(ZP(a.f3, i, j, k + 1), b3p - b3);
-    else if (sky) acc(ZP(a.f3, i, j, k), b3 - b3p);
-    else {
-        acc(ZP(a.f3, i, j, k + 1), b3p);
-        acc(ZP(a.f3, i, j, k), b3);
+        if (!(mx.val > mxo.val)) mx = mxo;   // mx = max2(mx, mxo)
+        if (!(mn.val < mno.val)) mn = mno;   // mn = min2(mn, mno)
+        const R a1p = X_(w1, i + 1, j, k), a1 = X_(w1, i, j, k);
+        const R a2p = Y_(w2, i, j + 1, k), a2 = Y_(w2, i, j, k);
+        R a3p, a3;
+        if (gnd) { a3p = Z_(w3, i, j, k + 1); a3 = -a3p; }
+        else if (sky) { a3 = Z_(w3, i, j, k); a3p = -a3; }
+        else { a3p = Z_(w3, i, j, k + 1); a3 = Z_(w3, i, j, k); }
+        const R hc = S_(h, i, j, k);
+        const R qp = (a1p < R(0) ? -a1p : R(0)) + (a1 > R(0) ? a1 : R(0)) + (a2p < R(0) ? -a2p : R(0)) +
+                     (a2 > R(0) ? a2 : R(0)) + (a3p < R(0) ? -a3p : R(0)) + (a3 > R(0) ? a3 : R(0)) + ep;
+        const R qn = (a1p > R(0) ? a1p : R(0)) + (a1 < R(0) ? -a1 : R(0)) + (a2p > R(0) ? a2p : R(0)) +
+                     (a2 < R(0) ? -a2 : R(0)) + (a3p > R(0) ? a3p : R(0)) + (a3 < R(0) ? -a3 : R(0)) + ep;
+        const R iqp = R(1) / qp, iqn = R(1) / qn;
+        const R cpb = S_(a.cp, i, j, k), cnb = S_(a.cn, i, j, k);
+        const R tpb = cpb * iqp, tnb = cnb * iqn;
+        const R qpb = -(cpb * S_(cp, i, j, k) * iqp);
+        const R qnb = -(cnb * S_(cn, i, j, k) * iqn);
+        // cp = (mx - xc)*hc/qp, cn = (xc - mn)*hc/qn
+        const R mxb = tpb * hc, mnb = -(tnb * hc);
+        atomicAdd(&L[rp.at(i, j, k)], -(tpb * hc) + tnb * hc);
+        atomicAdd(&L[(mx.src ? rp : rx).at(i + mx.di, j + mx.dj, k + mx.dk)], mxb);
+        atomicAdd(&L[(mn.src ? rp : rx).at(i + mn.di, j + mn.dj, k + mn.dk)], mnb);
+        // denominators: d pn(y) = -[y<0] dy, d pp(y) = [y>0] dy
+        const R b1p = -ind_neg(a1p) * qpb + ind_pos(a1p) * qnb;  // a1p enters qp as pn, qn as pp
+        const R b1 = ind_pos(a1) * qpb - ind_neg(a1) * qnb;
+        const R b2p = -ind_neg(a2p) * qpb + ind_pos(a2p) * qnb;
+        const R b2 = ind_pos(a2) * qpb - ind_neg(a2) * qnb;
+        const R b3p = -ind_neg(a3p) * qpb + ind_pos(a3p) * qnb;
+        const R b3 = ind_pos(a3) * qpb - ind_neg(a3) * qnb;
+        // wall faces of x and y are constants after the boundary condition
+        if (i < g.np) atomicAdd(&L[r1.at(i + 1, j, k)], b1p);
+        if (i > 1) atomicAdd(&L[r1.at(i, j, k)], b1);
+        if (j < g.mp) atomicAdd(&L[r2.at(i, j + 1, k)], b2p);
+        if (j > 1) atomicAdd(&L[r2.at(i, j, k)], b2);
+        if (gnd) atomicAdd(&L[r3.at(i, j, k + 1)], b3p - b3);
+        else if (sky) atomicAdd(&L[r3.at(i, j, k)], b3 - b3p);
+        else {
+            atomicAdd(&L[r3.at(i, j, k + 1)], b3p);
+            atomicAdd(&L[r3.at(i, j, k)], b3);
+        }
     }
+    __syncthreads();
+    flush_region<SCAL>(g, L, rp, a.psi, tid);
+    flush_region<SCAL>(g, L, rx, a.x0, tid);
+    flush_region<FACEX>(g, L, r1, a.f1, tid);
+    flush_region<FACEY>(g, L, r2, a.f2, tid);
+    flush_region<FACEZ>(g, L, r3, a.f3, tid);
 }
 
 // ---- pseudo-velocities and first-order fluxes (:72-202) --------------------
 // V = vdyfa(x1, x2, a, r) + sum of vcorra(a, b, y1, y2, r) terms; each vcorra
-// term carries its four b faces and the four cells of its y1/y2 sums.
+// term carries its four b faces and the four cells of its y1/y2 sums.  The
+// adjoints a block's faces scatter land within one cell (psi) or one face (u)
+// of the block's cells, so they are summed in LDS (indices below) first and
+// the block adds its region to the global arrays once.
+constexpr int PBX = BX, PBY = BY;
+constexpr int LP_N = 3 * (PBY + 2) * (PBX + 2);   // psi cells
+constexpr int LU_N = 2 * (PBY + 1) * (PBX + 1);   // faces of one direction
+constexpr int L_U1 = LP_N, L_U2 = LP_N + LU_N, L_U3 = LP_N + 2 * LU_N, L_N = LP_N + 3 * LU_N;
+
 struct Corr {
-    R b, y1, y2;
-    const R* bf[4];   // b = sum of these faces
-    R* bfb[4];        // their adjoints
-    R* cb[4];         // adjoints of the cells (plus pair: cells 0,1; minus pair: 2,3)
-    R cv[4];          // cell values
+    R b, y1, y2, iy2;   // b = sum of four faces; iy2 = 1/y2
+    int bf[4];          // LDS slots of those faces
+    int cb[4];          // LDS slots of the cells (plus pair 0,1; minus pair 2,3)
+    R sg[4];            // signs of the cell values
 };
 
-__device__ __forceinline__ void adj_vcorra(R vb, R a, const Corr& c, R r, R* ab) {
+__device__ __forceinline__ void adj_vcorra(R* L, R vb, R a, const Corr& c, R r, R& ab, R uf) {
     // val = -((((0.125*a)*b)*y1)*r / y2)
-    const R t1 = R(0.125) * a, t2 = t1 * c.b, t3 = t2 * c.y1, t4 = t3 * r, q = t4 / c.y2;
+    const R t1 = R(0.125) * a, t2 = t1 * c.b, t3 = t2 * c.y1, t4 = t3 * r, q = t4 * c.iy2;
     const R qb = -vb;
-    const R t4b = qb / c.y2;
-    const R y2b = -(qb * q / c.y2);
+    const R t4b = qb * c.iy2;
+    const R y2b = -(qb * q * c.iy2);
     const R t3b = t4b * r;
     const R y1b = t3b * t2;
     const R t2b = t3b * c.y1;
-    const R bb = t2b * t1;
-    const R t1b = t2b * c.b;
-    *ab += R(0.125) * t1b;
-    for (int q2 = 0; q2 < 4; q2++) acc(c.bfb[q2], bb);
+    const R bb = uf * (t2b * t1);
+    ab += R(0.125) * (t2b * c.b);
+#pragma unroll
+    for (int q2 = 0; q2 < 4; q2++) atomicAdd(&L[c.bf[q2]], bb);
     // y1 = (|c0| + |c1|) - (|c2| + |c3|), y2 = |c0| + |c1| + |c2| + |c3| + ep
-    acc(c.cb[0], (y1b + y2b) * sgn(c.cv[0]));
-    acc(c.cb[1], (y1b + y2b) * sgn(c.cv[1]));
-    acc(c.cb[2], (-y1b + y2b) * sgn(c.cv[2]));
-    acc(c.cb[3], (-y1b + y2b) * sgn(c.cv[3]));
+    atomicAdd(&L[c.cb[0]], (y1b + y2b) * c.sg[0]);
+    atomicAdd(&L[c.cb[1]], (y1b + y2b) * c.sg[1]);
+    atomicAdd(&L[c.cb[2]], (-y1b + y2b) * c.sg[2]);
+    atomicAdd(&L[c.cb[3]], (-y1b + y2b) * c.sg[3]);
 }
 
 // vb: adjoint of V at this face; fb: adjoint of fns = donor(x1, x2, V)
-__device__ __forceinline__ void adj_face_pseudo(R vb, R fb, R V, R x1, R x2, R a, R r, R* x1b,
-                                                R* x2b, R* ab_out, const Corr* cs, int ncorr) {
+__device__ __forceinline__ void adj_face_pseudo(R* L, R vb, R fb, R V, R x1, R x2, R a, R r,
+                                                int x1s, int x2s, int as, const Corr& c0,
+                                                const Corr& c1, bool two, R uf) {
     const R ep = Eps<R>::v;
     // fns = donor(x1, x2, V)
     vb += fb * (ind_pos(V) * x1 + ind_neg(V) * x2);
@@ -277,86 +384,136 @@ __device__ __forceinline__ void adj_face_pseudo(R vb, R fb, R V, R x1, R x2, R a
     R x2bv = fb * (V < R(0) ? V : R(0));
     // vdyfa: (|a| - a*a*r) * (|x2| - |x1|) / (|x2| + |x1| + ep)
     const R P = fabs(a) - a * a * r, N = fabs(x2) - fabs(x1), D = fabs(x2) + fabs(x1) + ep;
-    const R val = P * N / D;
-    const R tb = vb / D, Db = -(vb * val / D);
+    const R iD = R(1) / D;
+    const R val = P * N * iD;
+    const R tb = vb * iD, Db = -(vb * val * iD);
     const R Pb = tb * N, Nb = tb * P;
     x2bv += (Nb + Db) * sgn(x2);
     x1bv += (-Nb + Db) * sgn(x1);
     R ab = Pb * sgn(a) - Pb * r * (a + a);
-    for (int q = 0; q < ncorr; q++) adj_vcorra(vb, a, cs[q], r, &ab);
-    acc(x1b, x1bv);
-    acc(x2b, x2bv);
-    acc(ab_out, ab);
+    adj_vcorra(L, vb, a, c0, r, ab, uf);
+    if (two) adj_vcorra(L, vb, a, c1, r, ab, uf);
+    atomicAdd(&L[x1s], x1bv);
+    atomicAdd(&L[x2s], x2bv);
+    atomicAdd(&L[as], uf * ab);
 }
 
-__global__ void __launch_bounds__(256) k_adj_pseudo(Geo g, const R* __restrict__ u1,
-                                                    const R* __restrict__ u2,
-                                                    const R* __restrict__ u3,
-                                                    const R* __restrict__ psi,
-                                                    const R* __restrict__ h,
-                                                    const R* __restrict__ v1,
-                                                    const R* __restrict__ v2,
-                                                    const R* __restrict__ v3, R* __restrict__ ub1,
-                                                    R* __restrict__ ub2, R* __restrict__ ub3, Adj a) {
-    CELL_HEAD
+__global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __restrict__ u1,
+                                                          const R* __restrict__ u2,
+                                                          const R* __restrict__ u3,
+                                                          const R* __restrict__ psi,
+                                                          const R* __restrict__ h,
+                                                          const R* __restrict__ v1,
+                                                          const R* __restrict__ v2,
+                                                          const R* __restrict__ v3,
+                                                          R* __restrict__ ub1, R* __restrict__ ub2,
+                                                          R* __restrict__ ub3, Adj a) {
+    // LDS regions: psi cells (i0-1..i0+PBX, j0-1..j0+PBY, k-1..k+1); u1 faces
+    // (i0..i0+PBX, j0-1..j0+PBY-1, k-1..k); u2 faces (i0-1..i0+PBX-1,
+    // j0..j0+PBY, k-1..k); u3 faces (i0-1..i0+PBX-1, j0-1..j0+PBY-1, k..k+1)
+    __shared__ R L[L_N];
+    const int tid = threadIdx.y * PBX + threadIdx.x;
+    const int i0 = 1 + blockIdx.x * PBX, j0 = 1 + blockIdx.y * PBY, k = 1 + blockIdx.z;
+    for (int q = tid; q < L_N; q += PBX * PBY) L[q] = R(0);
+    __syncthreads();
+    const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    const bool valid = i <= g.np && j <= g.mp;
+    const bool gnd = k == 1, sky = k == g.lp;
     const bool zterms = !(gnd || sky);
     const R ep = Eps<R>::v;
+#define LPS(ii, jj, kk) ((((kk) - k + 1) * (PBY + 2) + ((jj) - j0 + 1)) * (PBX + 2) + ((ii) - i0 + 1))
+#define LU1(ii, jj, kk) (L_U1 + (((kk) - k + 1) * (PBY + 1) + ((jj) - j0 + 1)) * (PBX + 1) + ((ii) - i0))
+#define LU2(ii, jj, kk) (L_U2 + (((kk) - k + 1) * (PBY + 1) + ((jj) - j0)) * (PBX + 1) + ((ii) - i0 + 1))
+#define LU3(ii, jj, kk) (L_U3 + (((kk) - k) * (PBY + 1) + ((jj) - j0 + 1)) * (PBX + 1) + ((ii) - i0 + 1))
 #define AXV(ii, jj, kk) S_(psi, ii, jj, kk)
 #define CORR(c, F1, F2, F3, F4, BF1, BF2, BF3, BF4, I0, J0, K0, I1, J1, K1, I2, J2, K2, I3, J3, K3) \
     {                                                                                          \
-        c.bf[0] = &F1; c.bf[1] = &F2; c.bf[2] = &F3; c.bf[3] = &F4;                            \
-        c.bfb[0] = BF1; c.bfb[1] = BF2; c.bfb[2] = BF3; c.bfb[3] = BF4;                        \
-        c.b = *c.bf[0] + *c.bf[1] + *c.bf[2] + *c.bf[3];                                       \
-        c.cv[0] = AXV(I0, J0, K0); c.cv[1] = AXV(I1, J1, K1);                                  \
-        c.cv[2] = AXV(I2, J2, K2); c.cv[3] = AXV(I3, J3, K3);                                  \
-        c.cb[0] = SP(a.psi, I0, J0, K0); c.cb[1] = SP(a.psi, I1, J1, K1);                      \
-        c.cb[2] = SP(a.psi, I2, J2, K2); c.cb[3] = SP(a.psi, I3, J3, K3);                      \
-        c.y1 = (fabs(c.cv[0]) + fabs(c.cv[1])) - (fabs(c.cv[2]) + fabs(c.cv[3]));              \
-        c.y2 = fabs(c.cv[0]) + fabs(c.cv[1]) + fabs(c.cv[2]) + fabs(c.cv[3]) + ep;             \
+        c.bf[0] = BF1; c.bf[1] = BF2; c.bf[2] = BF3; c.bf[3] = BF4;                            \
+        c.b = F1 + F2 + F3 + F4;                                                               \
+        const R v0_ = AXV(I0, J0, K0), v1_ = AXV(I1, J1, K1);                                  \
+        const R v2_ = AXV(I2, J2, K2), v3_ = AXV(I3, J3, K3);                                  \
+        c.sg[0] = sgn(v0_); c.sg[1] = sgn(v1_); c.sg[2] = sgn(v2_); c.sg[3] = sgn(v3_);        \
+        c.cb[0] = LPS(I0, J0, K0); c.cb[1] = LPS(I1, J1, K1);                                  \
+        c.cb[2] = LPS(I2, J2, K2); c.cb[3] = LPS(I3, J3, K3);                                  \
+        c.y1 = (fabs(v0_) + fabs(v1_)) - (fabs(v2_) + fabs(v3_));                              \
+        c.y2 = fabs(v0_) + fabs(v1_) + fabs(v2_) + fabs(v3_) + ep;                             \
+        c.iy2 = R(1) / c.y2;                                                                   \
     }
-    if (i > 1) {
+    if (valid && i > 1) {
         const R hm = R(1) / (R(0.5) * (S_(h, i - 1, j, k) + S_(h, i, j, k)));
-        Corr cs[2];
-        CORR(cs[0], Y_(u2, i - 1, j, k), Y_(u2, i - 1, j + 1, k), Y_(u2, i, j + 1, k), Y_(u2, i, j, k),
-             YP(ub2, i - 1, j, k), YP(ub2, i - 1, j + 1, k), YP(ub2, i, j + 1, k), YP(ub2, i, j, k),
+        Corr c0, c1;
+        CORR(c0, Y_(u2, i - 1, j, k), Y_(u2, i - 1, j + 1, k), Y_(u2, i, j + 1, k), Y_(u2, i, j, k),
+             LU2(i - 1, j, k), LU2(i - 1, j + 1, k), LU2(i, j + 1, k), LU2(i, j, k),
              i - 1, j + 1, k, i, j + 1, k, i - 1, j - 1, k, i, j - 1, k)
         if (zterms)
-            CORR(cs[1], Z_(u3, i - 1, j, k), Z_(u3, i - 1, j, k + 1), Z_(u3, i, j, k + 1), Z_(u3, i, j, k),
-                 ZP(ub3, i - 1, j, k), ZP(ub3, i - 1, j, k + 1), ZP(ub3, i, j, k + 1), ZP(ub3, i, j, k),
+            CORR(c1, Z_(u3, i - 1, j, k), Z_(u3, i - 1, j, k + 1), Z_(u3, i, j, k + 1), Z_(u3, i, j, k),
+                 LU3(i - 1, j, k), LU3(i - 1, j, k + 1), LU3(i, j, k + 1), LU3(i, j, k),
                  i - 1, j, k + 1, i, j, k + 1, i - 1, j, k - 1, i, j, k - 1)
-        adj_face_pseudo(X_(a.v1, i, j, k), X_(a.f1, i, j, k), X_(v1, i, j, k), AXV(i - 1, j, k),
-                        AXV(i, j, k), X_(u1, i, j, k), hm, SP(a.psi, i - 1, j, k),
-                        SP(a.psi, i, j, k), XP(ub1, i, j, k), cs, zterms ? 2 : 1);
+        adj_face_pseudo(L, X_(a.v1, i, j, k), X_(a.f1, i, j, k), X_(v1, i, j, k), AXV(i - 1, j, k),
+                        AXV(i, j, k), X_(u1, i, j, k), hm, LPS(i - 1, j, k), LPS(i, j, k),
+                        LU1(i, j, k), c0, c1, zterms, a.uf);
     }
-    if (j > 1) {
+    if (valid && j > 1) {
         const R hm = R(1) / (R(0.5) * (S_(h, i, j - 1, k) + S_(h, i, j, k)));
-        Corr cs[2];
-        CORR(cs[0], X_(u1, i, j - 1, k), X_(u1, i, j, k), X_(u1, i + 1, j, k), X_(u1, i + 1, j - 1, k),
-             XP(ub1, i, j - 1, k), XP(ub1, i, j, k), XP(ub1, i + 1, j, k), XP(ub1, i + 1, j - 1, k),
+        Corr c0, c1;
+        CORR(c0, X_(u1, i, j - 1, k), X_(u1, i, j, k), X_(u1, i + 1, j, k), X_(u1, i + 1, j - 1, k),
+             LU1(i, j - 1, k), LU1(i, j, k), LU1(i + 1, j, k), LU1(i + 1, j - 1, k),
              i + 1, j - 1, k, i + 1, j, k, i - 1, j - 1, k, i - 1, j, k)
         if (zterms)
-            CORR(cs[1], Z_(u3, i, j - 1, k), Z_(u3, i, j, k), Z_(u3, i, j, k + 1), Z_(u3, i, j - 1, k + 1),
-                 ZP(ub3, i, j - 1, k), ZP(ub3, i, j, k), ZP(ub3, i, j, k + 1), ZP(ub3, i, j - 1, k + 1),
+            CORR(c1, Z_(u3, i, j - 1, k), Z_(u3, i, j, k), Z_(u3, i, j, k + 1), Z_(u3, i, j - 1, k + 1),
+                 LU3(i, j - 1, k), LU3(i, j, k), LU3(i, j, k + 1), LU3(i, j - 1, k + 1),
                  i, j - 1, k + 1, i, j, k + 1, i, j - 1, k - 1, i, j, k - 1)
-        adj_face_pseudo(Y_(a.v2, i, j, k), Y_(a.f2, i, j, k), Y_(v2, i, j, k), AXV(i, j - 1, k),
-                        AXV(i, j, k), Y_(u2, i, j, k), hm, SP(a.psi, i, j - 1, k),
-                        SP(a.psi, i, j, k), YP(ub2, i, j, k), cs, zterms ? 2 : 1);
+        adj_face_pseudo(L, Y_(a.v2, i, j, k), Y_(a.f2, i, j, k), Y_(v2, i, j, k), AXV(i, j - 1, k),
+                        AXV(i, j, k), Y_(u2, i, j, k), hm, LPS(i, j - 1, k), LPS(i, j, k),
+                        LU2(i, j, k), c0, c1, zterms, a.uf);
     }
-    if (!gnd) {
+    if (valid && !gnd) {
         const R hm = R(1) / (R(0.5) * (S_(h, i, j, k - 1) + S_(h, i, j, k)));
-        Corr cs[2];
-        CORR(cs[0], X_(u1, i, j, k - 1), X_(u1, i, j, k), X_(u1, i + 1, j, k), X_(u1, i + 1, j, k - 1),
-             XP(ub1, i, j, k - 1), XP(ub1, i, j, k), XP(ub1, i + 1, j, k), XP(ub1, i + 1, j, k - 1),
+        Corr c0, c1;
+        CORR(c0, X_(u1, i, j, k - 1), X_(u1, i, j, k), X_(u1, i + 1, j, k), X_(u1, i + 1, j, k - 1),
+             LU1(i, j, k - 1), LU1(i, j, k), LU1(i + 1, j, k), LU1(i + 1, j, k - 1),
              i + 1, j, k - 1, i + 1, j, k, i - 1, j, k - 1, i - 1, j, k)
-        CORR(cs[1], Y_(u2, i, j, k - 1), Y_(u2, i, j + 1, k - 1), Y_(u2, i, j + 1, k), Y_(u2, i, j, k),
-             YP(ub2, i, j, k - 1), YP(ub2, i, j + 1, k - 1), YP(ub2, i, j + 1, k), YP(ub2, i, j, k),
+        CORR(c1, Y_(u2, i, j, k - 1), Y_(u2, i, j + 1, k - 1), Y_(u2, i, j + 1, k), Y_(u2, i, j, k),
+             LU2(i, j, k - 1), LU2(i, j + 1, k - 1), LU2(i, j + 1, k), LU2(i, j, k),
              i, j + 1, k - 1, i, j + 1, k, i, j - 1, k - 1, i, j - 1, k)
-        adj_face_pseudo(Z_(a.v3, i, j, k), Z_(a.f3, i, j, k), Z_(v3, i, j, k), AXV(i, j, k - 1),
-                        AXV(i, j, k), Z_(u3, i, j, k), hm, SP(a.psi, i, j, k - 1),
-                        SP(a.psi, i, j, k), ZP(ub3, i, j, k), cs, 2);
+        adj_face_pseudo(L, Z_(a.v3, i, j, k), Z_(a.f3, i, j, k), Z_(v3, i, j, k), AXV(i, j, k - 1),
+                        AXV(i, j, k), Z_(u3, i, j, k), hm, LPS(i, j, k - 1), LPS(i, j, k),
+                        LU3(i, j, k), c0, c1, true, a.uf);
     }
 #undef CORR
 #undef AXV
+#undef LPS
+#undef LU1
+#undef LU2
+#undef LU3
+    __syncthreads();
+    // flush the regions; entries outside the arrays stay zero and are skipped
+    const int ih = g.ih;
+    for (int q = tid; q < LP_N; q += PBX * PBY) {
+        const R v = L[q];
+        if (v == R(0)) continue;
+        const int ii = i0 - 1 + q % (PBX + 2), jj = j0 - 1 + (q / (PBX + 2)) % (PBY + 2),
+                  kk = k - 1 + q / ((PBX + 2) * (PBY + 2));
+        if (ii <= g.np + ih && jj <= g.mp + ih) acc(SP(a.psi, ii, jj, kk), v);
+    }
+    for (int q = tid; q < LU_N; q += PBX * PBY) {
+        const int a0 = q % (PBX + 1), b0 = (q / (PBX + 1)) % (PBY + 1), c0 = q / ((PBX + 1) * (PBY + 1));
+        R v = L[L_U1 + q];
+        if (v != R(0)) {
+            const int ii = i0 + a0, jj = j0 - 1 + b0, kk = k - 1 + c0;
+            if (ii <= g.np + 1 + ih && jj <= g.mp + ih) acc(XP(ub1, ii, jj, kk), v);
+        }
+        v = L[L_U2 + q];
+        if (v != R(0)) {
+            const int ii = i0 - 1 + a0, jj = j0 + b0, kk = k - 1 + c0;
+            if (ii <= g.np + ih && jj <= g.mp + 1 + ih) acc(YP(ub2, ii, jj, kk), v);
+        }
+        v = L[L_U3 + q];
+        if (v != R(0)) {
+            const int ii = i0 - 1 + a0, jj = j0 - 1 + b0, kk = k + c0;
+            if (ii <= g.np + ih && jj <= g.mp + ih) acc(ZP(ub3, ii, jj, kk), v);
+        }
+    }
 }
 
 // ---- halo fills in reverse -------------------------------------------------
@@ -425,16 +582,18 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
         const R fh = sc - (i < np ? sbar_of(g, a, rhr, h, i + 1, j, k) : R(0));
         const R ul = X_(u1, i, j, k), uh = X_(u1, i + 1, j, k);
         xb_c += fl * (ul < R(0) ? ul : R(0)) + fh * (uh > R(0) ? uh : R(0));
-        X_(ub1, i, j, k) += fl * (ind_pos(ul) * S_(xb, i - 1, j, k) + ind_neg(ul) * xc);
-        if (i == np) X_(ub1, i + 1, j, k) += fh * (ind_pos(uh) * xc + ind_neg(uh) * S_(xb, i + 1, j, k));
+        X_(ub1, i, j, k) += a.uf * (fl * (ind_pos(ul) * S_(xb, i - 1, j, k) + ind_neg(ul) * xc));
+        if (i == np)
+            X_(ub1, i + 1, j, k) += a.uf * (fh * (ind_pos(uh) * xc + ind_neg(uh) * S_(xb, i + 1, j, k)));
     }
     {
         const R fl = (j > 1 ? sbar_of(g, a, rhr, h, i, j - 1, k) : R(0)) - sc;
         const R fh = sc - (j < mp ? sbar_of(g, a, rhr, h, i, j + 1, k) : R(0));
         const R ul = Y_(u2, i, j, k), uh = Y_(u2, i, j + 1, k);
         xb_c += fl * (ul < R(0) ? ul : R(0)) + fh * (uh > R(0) ? uh : R(0));
-        Y_(ub2, i, j, k) += fl * (ind_pos(ul) * S_(xb, i, j - 1, k) + ind_neg(ul) * xc);
-        if (j == mp) Y_(ub2, i, j + 1, k) += fh * (ind_pos(uh) * xc + ind_neg(uh) * S_(xb, i, j + 1, k));
+        Y_(ub2, i, j, k) += a.uf * (fl * (ind_pos(ul) * S_(xb, i, j - 1, k) + ind_neg(ul) * xc));
+        if (j == mp)
+            Y_(ub2, i, j + 1, k) += a.uf * (fh * (ind_pos(uh) * xc + ind_neg(uh) * S_(xb, i, j + 1, k)));
     }
     {
         // z faces 2..lp; the ground cell counts face 2 twice, the sky cell face lp
@@ -443,7 +602,7 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
             const R fl = c1 * sbar_of(g, a, rhr, h, i, j, k - 1) - c2 * sc;
             const R ul = Z_(u3, i, j, k);
             xb_c += fl * (ul < R(0) ? ul : R(0));
-            Z_(ub3, i, j, k) += fl * (ind_pos(ul) * S_(xb, i, j, k - 1) + ind_neg(ul) * xc);
+            Z_(ub3, i, j, k) += a.uf * (fl * (ind_pos(ul) * S_(xb, i, j, k - 1) + ind_neg(ul) * xc));
         }
         if (!sky) {
             const R c1 = gnd ? R(2) : R(1), c2 = k + 1 == lp ? R(2) : R(1);
@@ -538,17 +697,15 @@ int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x
         f[d] = (R*)S.f[d];
     }
     // the stage sequence of the standard non-oscillatory step
-    HIPCHK(hipMemcpyAsync(xb, x_in, sb, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(x0, x_in, sb, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(x_out, x_in, sb, hipMemcpyDeviceToDevice, s));
-    HIPCHK(St::putbcinxh(g, xb, bcx, bcy, s));
+    k_prep_x<<<dim3((g.np + 2 * g.ih + 63) / 64, (g.mp + 2 * g.ih + 3) / 4, g.lp + 2 * g.ih),
+               dim3(64, 4, 1), 0, s>>>(g, x_in, bcx, bcy, xb, x0);
+    HIPCHK(hipGetLastError());
     HIPCHK(St::upwind(g, u1, u2, u3, xb, psi, rhr, h, s));
     HIPCHK(St::halo_xy(g, psi, St::MIRROR, 1, s));
     HIPCHK(St::pseudo_vel(g, MPDATA_STANDARD, u1, u2, u3, psi, h, v[0], v[1], v[2], fns[0],
                           fns[1], fns[2], s));
     HIPCHK(St::face_bc(g, v[0], v[1], v[2], s));
     HIPCHK(St::face_bc(g, fns[0], fns[1], fns[2], s));
-    HIPCHK(St::halo_xy(g, x0, St::COPY, 0, s));
     HIPCHK(St::limiter(g, x0, psi, h, fns[0], fns[1], fns[2], cp, cn, s));
     HIPCHK(St::halo_xy(g, cp, St::ZERO, 1, s));
     HIPCHK(St::halo_xy(g, cn, St::ZERO, 1, s));
@@ -562,7 +719,7 @@ int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x
 
 int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double* u2,
                         const double* u3, const double* rhr, const double* h, double* xbar,
-                        double* ubar1, double* ubar2, double* ubar3, void* stream) {
+                        double* ubar1, double* ubar2, double* ubar3, double ufac, void* stream) {
     int rc = check_adj(c);
     if (rc) return rc;
     if (slot < 0 || slot >= (int)c->tape.size() || !c->tape[slot].xb)
@@ -574,24 +731,34 @@ int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double*
         if ((rc = ensure(c, &c->adj[q], A_KIND[q], 8))) return rc;
     Adj a{(R*)c->adj[A_PSI], (R*)c->adj[A_X0], (R*)c->adj[A_CP], (R*)c->adj[A_CN],
           (R*)c->adj[A_SUM], (R*)c->adj[A_V1], (R*)c->adj[A_V2], (R*)c->adj[A_V3],
-          (R*)c->adj[A_F1], (R*)c->adj[A_F2], (R*)c->adj[A_F3]};
+          (R*)c->adj[A_F1], (R*)c->adj[A_F2], (R*)c->adj[A_F3], ufac};
     const TapeSlot& t = c->tape[slot];
     hipStream_t s = (hipStream_t)stream;
-    for (int q = 0; q < A_N; q++) {
-        const long long n = field_len(g, A_KIND[q]);
-        k_zero<<<(int)std::min<long long>(1024, (n + 255) / 256), 256, 0, s>>>((R*)c->adj[q], n);
+    // arrays that receive scattered contributions start at zero; a.sum and
+    // the pseudo-velocity adjoints are written before they are read
+    {
+        const int zq[7] = {A_PSI, A_X0, A_CP, A_CN, A_F1, A_F2, A_F3};
+        ZeroList z;
+        long long nmax = 0;
+        for (int q = 0; q < 7; q++) {
+            z.p[q] = (R*)c->adj[zq[q]];
+            z.n[q] = field_len(g, A_KIND[zq[q]]);
+            nmax = std::max(nmax, z.n[q]);
+        }
+        k_zero<<<dim3((unsigned)std::min<long long>(512, (nmax + 255) / 256), 7), 256, 0, s>>>(z);
     }
     HIPCHK(hipGetLastError());
     k_adj_update<<<cgrid(g), cblock(), 0, s>>>(g, xbar, rhr, h, a);
-    k_adj_flux<<<cgrid(g), cblock(), 0, s>>>(g, (const R*)t.v[0], (const R*)t.v[1],
+    k_adj_flux<<<dim3((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, g.lp), dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.v[0], (const R*)t.v[1],
                                              (const R*)t.v[2], (const R*)t.vl[0],
                                              (const R*)t.vl[1], (const R*)t.vl[2],
                                              (const R*)t.cp, (const R*)t.cn, (const R*)t.psi, rhr, a);
-    k_adj_limiter<<<cgrid(g), cblock(), 0, s>>>(g, (const R*)t.x0, (const R*)t.psi, h,
+    k_adj_limiter<<<dim3((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, g.lp), dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.x0, (const R*)t.psi, h,
                                                 (const R*)t.fns[0], (const R*)t.fns[1],
                                                 (const R*)t.fns[2], (const R*)t.cp,
                                                 (const R*)t.cn, a);
-    k_adj_pseudo<<<cgrid(g), cblock(), 0, s>>>(g, u1, u2, u3, (const R*)t.psi, h,
+    k_adj_pseudo<<<dim3((g.np + PBX - 1) / PBX, (g.mp + PBY - 1) / PBY, g.lp), dim3(PBX, PBY, 1), 0,
+                   s>>>(g, u1, u2, u3, (const R*)t.psi, h,
                                                (const R*)t.v[0], (const R*)t.v[1],
                                                (const R*)t.v[2], ubar1, ubar2, ubar3, a);
     const int wall_y = g.mp + 2 * g.ih, wall_x = g.np + 2 * g.ih, planes = g.lp + 2 * g.ih;

@@ -222,14 +222,14 @@ class Domain:
                                         self._p(x_out, 0, "x_out"), pu1, pu2, pu3, prhr, ph,
                                         pbx, pby, self._stream()))
 
-    def adj_step(self, slot, u1, u2, u3, rhr, h, xbar, ubar1, ubar2, ubar3):
+    def adj_step(self, slot, u1, u2, u3, rhr, h, xbar, ubar1, ubar2, ubar3, ufac=1.0):
         """xbar (adjoint of the recorded step's output) <- adjoint of its
-        input, in place; ubar1..3 += the velocity adjoint."""
+        input, in place; ubar1..3 += ufac * the velocity adjoint."""
         pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
         check(LIB.mpdata_adj_step_f64(self._ctx, int(slot), pu1, pu2, pu3, prhr, ph,
                                       self._p(xbar, 0, "xbar"), self._p(ubar1, 1, "ubar1"),
                                       self._p(ubar2, 2, "ubar2"), self._p(ubar3, 3, "ubar3"),
-                                      self._stream()))
+                                      ctypes.c_double(ufac), self._stream()))
 
     # -- velocity predictor ----------------------------------------------------
     def _levels(self, t, name):

@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched-ensemble measurement (64 tracers on uneven grids)")
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint cost measurement (128x128x64, 50 steps)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of one field (counter calibration)")
     return ap.parse_args()
@@ -162,6 +164,11 @@ def main():
         torch.cuda.empty_cache()
         ens = ensemble_bench(a.ensemble_steps)
 
+    adj = None
+    if not a.no_adjoint and a.dtype == "f64":
+        torch.cuda.empty_cache()
+        adj = adjoint_bench()
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -190,6 +197,7 @@ def main():
         "cpu_baseline": cpu,
         "unfused_stages": unfused,
         "ensemble": ens,
+        "adjoint": adj,
         "checked_fused_equals_stages": checked,
         "wall_s_timed": wall,
     }
@@ -254,6 +262,58 @@ def ensemble_bench(steps, nmember=64, seed=4):
             "sequential_ms_per_step": ms_seq / steps,
             "sequential_value": cells * steps / (ms_seq / 1e3),
             "speedup_graph_vs_sequential": ms_seq / ms_graph}
+
+
+def adjoint_bench(nsteps=50, ckpt=10, reps=3):
+    """Gradient of J = 0.5 |x_T - obs|^2 after `nsteps` standard
+    non-oscillatory steps at 128x128x64 under the reversing deformational
+    flow, with respect to x0 and the face velocities, checkpoints every
+    `ckpt` steps: time of forward + backward against one plain forward of
+    the same steps (both include forming each step's velocity)."""
+    import torch
+
+    from advectionhpctester_amd import adjoint, mpdata, testdata
+    c = testdata.adjoint_case(128, 128, 64, nsteps=nsteps)
+    f = c.pop("factors")
+    dev = torch.device("cuda:0")
+    obs = torch.from_numpy(c.pop("obs")).to(dev)
+    d = {k: torch.from_numpy(v).to(dev) for k, v in c.items()}
+    rest = (d["rhr"], d["h"], d["bcx"], d["bcy"])
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with mpdata.Domain(128, 128, 64) as dom:
+        def plain():
+            x = d["x"].clone()
+            for n in range(nsteps):
+                dom.step(x, d["u1"] * f[n], d["u2"] * f[n], d["u3"] * f[n], *rest,
+                         variant=mpdata.STANDARD, iters=2, nonos=1)
+            return x
+
+        def fwd_bwd():
+            leaves = [d[k].clone().requires_grad_(True) for k in ("x", "u1", "u2", "u3")]
+            xT = adjoint.advect(*leaves, dom, *rest, nsteps, ckpt=ckpt, factors=f)
+            r = mpdata.interior(xT - obs)
+            (0.5 * (r * r).sum()).backward()
+            return leaves
+
+        def timed(fn):
+            fn()
+            torch.cuda.synchronize()
+            best = None
+            for _ in range(reps):
+                ev0.record()
+                fn()
+                ev1.record()
+                torch.cuda.synchronize()
+                t = ev0.elapsed_time(ev1)
+                best = t if best is None else min(best, t)
+            return best
+
+        t_fwd = timed(plain)
+        t_fb = timed(fwd_bwd)
+    return {"grid": [128, 128, 64], "steps": nsteps, "checkpoint_every": ckpt,
+            "variant": "standard", "iters": 2, "nonos": 1,
+            "forward_ms": t_fwd, "forward_backward_ms": t_fb, "ratio": t_fb / t_fwd,
+            "target_ratio": 4.0}
 
 
 def cpu_baseline(case, dims, dtype, steps):

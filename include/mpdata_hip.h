@@ -256,8 +256,10 @@ int mpdata_tl_step_f64(mpdata_ctx *ctx, int variant, int iters, int nonos, doubl
  * mpdata_adj_record_f64 runs one forward step x_in -> x_out (x_out as
  * mpdata_step leaves x) keeping its intermediates in `slot`;
  * mpdata_adj_step_f64 maps xbar (adjoint of the interior of x_out) to the
- * adjoint of the interior of x_in, in place, and ADDS the velocity adjoint
- * to ubar1..ubar3 (zero them before the first step). */
+ * adjoint of the interior of x_in, in place, and ADDS ufac times the
+ * velocity adjoint to ubar1..ubar3 (zero them before the first step; ufac
+ * lets a flow u(t) = f(t) u accumulate the gradient of its shape u).
+ * x_out receives the new interior; its halo is not defined. */
 int mpdata_adj_reserve(mpdata_ctx *ctx, int nslots);
 int mpdata_adj_record_f64(mpdata_ctx *ctx, int slot, const double *x_in, double *x_out,
                           const double *u1, const double *u2, const double *u3,
@@ -265,7 +267,7 @@ int mpdata_adj_record_f64(mpdata_ctx *ctx, int slot, const double *x_in, double 
                           const double *bcy, void *stream);
 int mpdata_adj_step_f64(mpdata_ctx *ctx, int slot, const double *u1, const double *u2,
                         const double *u3, const double *rhr, const double *h, double *xbar,
-                        double *ubar1, double *ubar2, double *ubar3, void *stream);
+                        double *ubar1, double *ubar2, double *ubar3, double ufac, void *stream);
 
 /* ---- ensembles: many domains in one launch ---------------------------------
  * No counterpart in the reference (its batching is many fields on one grid,
