@@ -422,7 +422,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     // register queues of the source column; during step K they hold
     //   xq1..3 = x(K-1..K+1), u1a0/1 = u1(i,K-1..K), u1b0/1 = u1(i+1,K-1..K),
     //   u2a/u2b likewise at j and j+1, u3q0..2 = faces K-1..K+1,
-    //   hq0..2, rq0..2 = h, rhr(K-2..K), xaq1..2 = xant(K-2..K-1),
+    //   hq1..2 = h(K-1..K), rq2 = rhr(K), xaq1..2 = xant(K-2..K-1),
     //   s1q/s2q = pair sums of planes K-2..K-1, v3q1..2 = V3 faces K-2..K-1,
     //   xmx0/xmn0 = bounds of x for the limiter on plane K-1
     T xq1 = x[bs + SXY * kc(Ks - 1)], xq2 = x[bs + SXY * kc(Ks)], xq3 = x[bs + SXY * kc(Ks + 1)];
@@ -431,8 +431,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     T u2a0 = u2[b2 + YXY * kc(Ks - 1)], u2a1 = u2[b2 + YXY * kc(Ks)];
     T u2b0 = u2[b2 + o.yx + YXY * kc(Ks - 1)], u2b1 = u2[b2 + o.yx + YXY * kc(Ks)];
     T u3q0 = u3[bs + SXY * fc(Ks - 1)], u3q1 = u3[bs + SXY * fc(Ks)], u3q2 = u3[bs + SXY * fc(Ks + 1)];
-    T hq0 = h[bs + SXY * kc(Ks - 2)], hq1 = h[bs + SXY * kc(Ks - 1)], hq2 = h[bs + SXY * kc(Ks)];
-    T rq0 = rhr[bs + SXY * kc(Ks - 2)], rq1 = rhr[bs + SXY * kc(Ks - 1)], rq2 = rhr[bs + SXY * kc(Ks)];
+    T hq1 = h[bs + SXY * kc(Ks - 1)], hq2 = h[bs + SXY * kc(Ks)];
+    T rq2 = rhr[bs + SXY * kc(Ks)];
     T xaq1 = T(0), xaq2 = T(0);
     T s1q1 = T(0), s1q2 = T(0), s2q1 = T(0), s2q2 = T(0);
     T v3q1 = T(0), v3q2 = T(0);
@@ -566,6 +566,15 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         const T n_u3 = u3[bs + SXY * fc(Kn + 1)];
         const T n_h = h[bs + SXY * kc(Kn)], n_r = rhr[bs + SXY * kc(Kn)];
         __syncthreads();
+        // h and rhr of the output plane Q = K-2, needed in stage D: loaded
+        // here (cache hits) rather than carried in registers for two planes
+        const int Q = K - 2;
+        const bool D_on = do_d && Q >= k0 && Q <= k1;
+        T hQ = T(1), rQ = T(1);
+        if (D_on) {
+            hQ = h[bs + SXY * Q];
+            rQ = rhr[bs + SXY * Q];
+        }
         // ---- C: cp, cn on plane P ------------------------------------------
         if (BC_on) {
             const bool gnd = P == 1, sky = P == lp;
@@ -596,8 +605,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         }
         // ---- D: the new x on plane Q=K-2 (neighbour data of plane Q was
         //      written in the previous step; own cp/cn of Q+1 just now) ----
-        const int Q = K - 2;
-        if (do_d && Q >= k0 && Q <= k1) {
+        if (D_on) {
             const bool gnd = Q == 1, sky = Q == lp;
 #define CP(pl, a, b) scp[(pl) % 3][(b) + 2][(a) + 2]
 #define CN(pl, a, b) scn[(pl) % 3][(b) + 2][(a) + 2]
@@ -623,8 +631,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
 #undef CP
 #undef CN
             const int c = bs + SXY * Q;
-            const T r = rq0;
-            xo[c] = r * (xaq1 / r - (F1p - F1 + F2p - F2 + F3p - F3) / hq0);
+            const T r = rQ;
+            xo[c] = r * (xaq1 / r - (F1p - F1 + F2p - F2 + F3p - F3) / hQ);
             if (gi == 1 || gi == np || gj == 1 || gj == mp) {
                 const T xe = x[c];
                 if (gi == 1) xo[c - 1] = xe;
@@ -639,8 +647,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         u1a0 = u1a1; u1a1 = n_u1a; u1b0 = u1b1; u1b1 = n_u1b;
         u2a0 = u2a1; u2a1 = n_u2a; u2b0 = u2b1; u2b1 = n_u2b;
         u3q0 = u3q1; u3q1 = u3q2; u3q2 = n_u3;
-        hq0 = hq1; hq1 = hq2; hq2 = n_h;
-        rq0 = rq1; rq1 = rq2; rq2 = n_r;
+        hq1 = hq2; hq2 = n_h;
+        rq2 = n_r;
         xaq1 = xaq2; xaq2 = xa_new;
         s1q1 = s1q2; s1q2 = s1n;
         s2q1 = s2q2; s2q2 = s2n;
@@ -732,9 +740,11 @@ __global__ void k_copy_untouched_halo(Geo g, const T* __restrict__ x, T* __restr
 
 constexpr int FTX = 32, FTY = 8;
 
-// Kernel generation (1 = per-region strided loops, 2 = fixed positions) and
-// k-chunk length; MPDATA_FUSED_GEN / MPDATA_FUSED_CHUNK override them for
-// tuning runs.
+// Kernel generation (1 = per-region strided loops, 2 = fixed positions),
+// tile configuration of generation 2 (0 = 32x16 lanes at 4 waves per SIMD,
+// the fastest at 512x512x256; 1-4 the alternatives in the switch below) and
+// k-chunk length; MPDATA_FUSED_GEN / _CFG / _CHUNK override them for tuning
+// runs.
 int fused_generation() {
     const char* e = getenv("MPDATA_FUSED_GEN");
     return e && atoi(e) == 1 ? 1 : 2;
@@ -914,11 +924,11 @@ hipError_t Fused<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const 
     }
     hipError_t e = hipSuccess;
     switch (fused_config()) {
-        case 1: e = launch2<T, 32, 16, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+        case 1: e = launch2<T, 32, 16, 2>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
         case 2: e = launch2<T, 32, 32, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
         case 3: e = launch2<T, 64, 16, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
         case 4: e = launch2<T, 16, 32, 2>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
-        default: e = launch2<T, 32, 16, 2>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+        default: e = launch2<T, 32, 16, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
     }
     if (e != hipSuccess) return e;
     k_copy_untouched_halo<T><<<256, 256, 0, s>>>(g, x, xout);
