@@ -49,6 +49,19 @@ struct Off {
 };
 
 __device__ __forceinline__ int mirror(int i, int n) { return i == 0 ? 2 : (i == n + 1 ? n - 1 : i); }
+
+// Element at byte offset `off` from `p`, where p is uniform over the
+// workgroup (an array's plane) and off a per-lane 32-bit offset: the
+// global_load/store saddr form (scalar base + 32-bit vector offset), with no
+// per-lane 64-bit address arithmetic.
+template <typename T>
+__device__ __forceinline__ T ldo(const T* p, uint32_t off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) + off);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T* p, uint32_t off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off) = v;
+}
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
 template <typename T, int TX, int TY>
@@ -317,14 +330,21 @@ k_fused_gauge(Geo g, Off o, const T* __restrict__ x, const T* __restrict__ u1,
 // a position needs only at its own column (x, u, h, rhr of the planes around
 // K, its xant, V3, pair sums) lives in a register queue, and only values
 // read by in-plane neighbours go through LDS:
-//   sx   x on plane K with a 3-cell ring (inflow halo values)
-//   sxa  xant, planes K-1 and K
+//   sx   x on planes K and K+1 with a 3-cell ring (inflow halo values)
+//   sxa  xant, planes K-2 .. K
 //   sv1, sv2   V1 and V2, planes K-2 and K-1
 //   scp, scn   cp and cn, planes K-3 .. K-1
 // The z cross terms of V1 and V2 use pair sums formed one plane at a time:
 // rat4(xa(i-1,k-1), xa(i,k-1), xa(i-1,k+1), xa(i,k+1)) =
 //   ((xa(i,k+1)+xa(i-1,k+1)) - (xa(i,k-1)+xa(i-1,k-1)))*0.25,
 // the same operations in the same order as the reference.
+//
+// Two barriers per plane: stage A reads the x plane that the previous step
+// wrote and writes the next one; B reads the xant A wrote; C/D read the V
+// that B wrote.  A slot of a ring is rewritten only after a barrier that
+// every reader of its previous plane has passed (three xant slots, because
+// A of plane K runs while slower threads may still be in C of plane K-1,
+// reading xant K-2).
 //
 // The grid is persistent: workgroups loop over (tile, k-chunk) work items,
 // a chunk of output planes [k0,k1] recomputing xant from k0-2 and cp,cn,V
@@ -341,8 +361,8 @@ struct XSrc {  // where one sx cell comes from: x, bcx, bcy or nothing
 // LDS of one workgroup of the fixed-position kernel.
 template <typename T, int RX, int RY>
 struct FusedLds {
-    T sx[RY + 2][RX + 2];
-    T sxa[2][RY][RX];
+    T sx[2][RY + 2][RX + 2];
+    T sxa[3][RY][RX];
     T sv1[2][RY][RX];
     T sv2[2][RY][RX];
     T scp[3][RY][RX];
@@ -410,8 +430,15 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     // one-cell halo ring (cp_scnd_last_to_halo_xy_fulld), clamped beyond
     const int ai = clampi(mirror(gi, np), 1, np), aj = clampi(mirror(gj, mp), 1, mp);
     const int sai = ai - i0 + 3, saj = aj - j0 + 3;  // its sx coordinates
-    const int bs = o.s(ai, aj, 0), b1 = o.f1(ai, aj, 0), b2 = o.f2(ai, aj, 0);
+    const int bs = o.s(ai, aj, 0);
     const XSrc<T> own = in_dom ? XSrc<T>{x, bs, SXY} : xsrc(gi, gj);
+    // byte offsets of the source column inside a plane: so for cell arrays,
+    // z-faces and y-faces (whose in-plane strides equal the cell ones),
+    // fo for x-faces; planes are addressed by uniform pointers
+    const uint32_t so = uint32_t(o.s(ai, aj, 1 - o.ih)) * uint32_t(sizeof(T));
+    const uint32_t fo = uint32_t(o.f1(ai, aj, 1 - o.ih)) * uint32_t(sizeof(T));
+    const int ihm = o.ih - 1;
+    auto pl = [&](auto* a, int stride, int k) { return a + (long long)stride * (k + ihm); };
     const XSrc<T> ring = t < NRING ? xsrc(i0 + rsi - 3, j0 + rsj - 3) : XSrc<T>{nullptr, 0, 0};
     // x-neighbour availability for the in-plane max/min (edge copies
     // duplicate the centre and do not change it)
@@ -425,23 +452,42 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     //   hq1..2 = h(K-1..K), rq2 = rhr(K), xaq1..2 = xant(K-2..K-1),
     //   s1q/s2q = pair sums of planes K-2..K-1, v3q1..2 = V3 faces K-2..K-1,
     //   xmx0/xmn0 = bounds of x for the limiter on plane K-1
-    T xq1 = x[bs + SXY * kc(Ks - 1)], xq2 = x[bs + SXY * kc(Ks)], xq3 = x[bs + SXY * kc(Ks + 1)];
-    T u1a0 = u1[b1 + FXY * kc(Ks - 1)], u1a1 = u1[b1 + FXY * kc(Ks)];
-    T u1b0 = u1[b1 + 1 + FXY * kc(Ks - 1)], u1b1 = u1[b1 + 1 + FXY * kc(Ks)];
-    T u2a0 = u2[b2 + YXY * kc(Ks - 1)], u2a1 = u2[b2 + YXY * kc(Ks)];
-    T u2b0 = u2[b2 + o.yx + YXY * kc(Ks - 1)], u2b1 = u2[b2 + o.yx + YXY * kc(Ks)];
-    T u3q0 = u3[bs + SXY * fc(Ks - 1)], u3q1 = u3[bs + SXY * fc(Ks)], u3q2 = u3[bs + SXY * fc(Ks + 1)];
-    T hq1 = h[bs + SXY * kc(Ks - 1)], hq2 = h[bs + SXY * kc(Ks)];
-    T rq2 = rhr[bs + SXY * kc(Ks)];
+    T xq1 = ldo(pl(x, SXY, kc(Ks - 1)), so), xq2 = ldo(pl(x, SXY, kc(Ks)), so);
+    T xq3 = ldo(pl(x, SXY, kc(Ks + 1)), so);
+    T u1a0 = ldo(pl(u1, FXY, kc(Ks - 1)), fo), u1a1 = ldo(pl(u1, FXY, kc(Ks)), fo);
+    T u1b0 = ldo(pl(u1, FXY, kc(Ks - 1)) + 1, fo), u1b1 = ldo(pl(u1, FXY, kc(Ks)) + 1, fo);
+    T u2a0 = ldo(pl(u2, YXY, kc(Ks - 1)), so), u2a1 = ldo(pl(u2, YXY, kc(Ks)), so);
+    T u2b0 = ldo(pl(u2, YXY, kc(Ks - 1)) + o.yx, so), u2b1 = ldo(pl(u2, YXY, kc(Ks)) + o.yx, so);
+    T u3q0 = ldo(pl(u3, SXY, fc(Ks - 1)), so), u3q1 = ldo(pl(u3, SXY, fc(Ks)), so);
+    T u3q2 = ldo(pl(u3, SXY, fc(Ks + 1)), so);
+    T hq1 = ldo(pl(h, SXY, kc(Ks - 1)), so), hq2 = ldo(pl(h, SXY, kc(Ks)), so);
+    T rq2 = ldo(pl(rhr, SXY, kc(Ks)), so);
     T xaq1 = T(0), xaq2 = T(0);
     T s1q1 = T(0), s1q2 = T(0), s2q1 = T(0), s2q2 = T(0);
     T v3q1 = T(0), v3q2 = T(0);
     T xmx0 = T(0), xmn0 = T(0), xmx1 = T(0), xmn1 = T(0);  // x in-plane max/min, planes K-1, K
 
+    // x of the first plane into LDS; the barrier also orders this item
+    // after every LDS read of the previous one
+    sx[Ks & 1][pj + 1][pi + 1] = in_dom ? xq2 : ld(own, Ks);
+    if (t < NRING) sx[Ks & 1][rsj][rsi] = ld(ring, Ks);
+    __syncthreads();
+
+    uint32_t so_k = so, fo_k = fo;
     for (int K = Ks; K <= Ke; ++K) {
-        const bool A_on = K <= Alast;
-        // x ring cell of plane K (own cell comes from the queue)
-        const T xr = A_on ? ld(ring, K) : T(0);
+        // the column offsets are opaque to the optimiser at each plane, so
+        // loads keep the saddr form (plane pointer + offset) instead of
+        // becoming per-lane 64-bit pointer inductions
+        asm volatile("" : "+v"(so_k), "+v"(fo_k));
+        const uint32_t so = so_k, fo = fo_k;
+        const bool A_on = K <= Alast, An_on = K + 1 <= Alast;
+        // x of plane K+1 for the LDS: ring cell, and the own cell where it
+        // is not in the queue (positions outside the domain)
+        T xr = T(0), xown = T(0);
+        if (An_on) {
+            if (t < NRING) xr = ld(ring, K + 1);
+            if (!in_dom) xown = ld(own, K + 1);
+        }
         // neighbour-column inputs of V1/V2 on plane P=K-1, loaded before
         // the barriers so that their latency overlaps stage A
         const int P = K - 1;
@@ -449,32 +495,26 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         const bool zt = !(P == 1 || P == lp);
         T nv1_h = T(0), nv1_u2a = T(0), nv1_u2b = T(0), nv1_u3a = T(0), nv1_u3b = T(0);
         T nv2_h = T(0), nv2_u1a = T(0), nv2_u1b = T(0), nv2_u3a = T(0), nv2_u3b = T(0);
-        if (BC_on && do_v1) {
-            const int cm = bs - 1 + SXY * P;  // (gi-1, gj, P)
-            const int c2 = b2 - 1 + YXY * P;
-            nv1_h = h[cm];
-            nv1_u2a = u2[c2];
-            nv1_u2b = u2[c2 + o.yx];
-            if (zt) { nv1_u3a = u3[cm]; nv1_u3b = u3[cm + SXY]; }
+        if (BC_on && do_v1) {  // column (gi-1, gj)
+            nv1_h = ldo(pl(h, SXY, P) - 1, so);
+            nv1_u2a = ldo(pl(u2, YXY, P) - 1, so);
+            nv1_u2b = ldo(pl(u2, YXY, P) - 1 + o.yx, so);
+            if (zt) { nv1_u3a = ldo(pl(u3, SXY, P) - 1, so); nv1_u3b = ldo(pl(u3, SXY, P + 1) - 1, so); }
         }
-        if (BC_on && do_v2) {
-            const int cm = bs - o.sx + SXY * P;  // (gi, gj-1, P)
-            const int c1 = b1 - o.fx + FXY * P;
-            nv2_h = h[cm];
-            nv2_u1a = u1[c1];
-            nv2_u1b = u1[c1 + 1];
-            if (zt) { nv2_u3a = u3[cm]; nv2_u3b = u3[cm + SXY]; }
+        if (BC_on && do_v2) {  // column (gi, gj-1)
+            nv2_h = ldo(pl(h, SXY, P) - o.sx, so);
+            nv2_u1a = ldo(pl(u1, FXY, P) - o.fx, fo);
+            nv2_u1b = ldo(pl(u1, FXY, P) - o.fx + 1, fo);
+            if (zt) {
+                nv2_u3a = ldo(pl(u3, SXY, P) - o.sx, so);
+                nv2_u3b = ldo(pl(u3, SXY, P + 1) - o.sx, so);
+            }
         }
 
-        __syncthreads();  // previous step's LDS readers are done
         // ---- A: xant on plane K ------------------------------------------
-        if (A_on) {
-            sx[pj + 1][pi + 1] = in_dom ? xq2 : ld(own, K);
-            if (t < NRING) sx[rsj][rsi] = xr;
-        }
-        __syncthreads();
         T xa_new = T(0);
         if (A_on) {
+            auto& sx = L.sx[K & 1];
             const bool gnd = K == 1, sky = K == lp;
             T hinv = T(1) / hq2;
             T xc = xq2;
@@ -494,7 +534,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 f3 = donor(xq1, xc, u3q1);
             }
             xa_new = rq2 * (xc - (f1p - f1 + f2p - f2 + f3p - f3) * hinv);
-            sxa[K & 1][pj][pi] = xa_new;
+            sxa[K % 3][pj][pi] = xa_new;
             // bounds of x for the limiter on plane K: the 7-point stencil,
             // where out-of-domain neighbours are edge copies of the centre
             // (cp_last_to_halo_xy) and leave max/min unchanged
@@ -508,8 +548,12 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             xmx1 = m;
             xmn1 = n;
         }
+        if (An_on) {
+            sx[(K + 1) & 1][pj + 1][pi + 1] = in_dom ? xq3 : xown;
+            if (t < NRING) sx[(K + 1) & 1][rsj][rsi] = xr;
+        }
         __syncthreads();
-#define XA(pl, a, b) sxa[(pl) & 1][(b) + 2][(a) + 2]
+#define XA(pl, a, b) sxa[(pl) % 3][(b) + 2][(a) + 2]
         // ---- B: pair sums, V1/V2 on plane P=K-1, V3 on face K --------------
         T s1n = T(0), s2n = T(0), v3n = T(0);
         if (A_on) {
@@ -560,11 +604,11 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         // prefetch the source-column inputs of plane K+1 (consumed at the
         // end of this step)
         const int Kn = K + 1;
-        const T n_x = x[bs + SXY * kc(Kn + 1)];
-        const T n_u1a = u1[b1 + FXY * kc(Kn)], n_u1b = u1[b1 + 1 + FXY * kc(Kn)];
-        const T n_u2a = u2[b2 + YXY * kc(Kn)], n_u2b = u2[b2 + o.yx + YXY * kc(Kn)];
-        const T n_u3 = u3[bs + SXY * fc(Kn + 1)];
-        const T n_h = h[bs + SXY * kc(Kn)], n_r = rhr[bs + SXY * kc(Kn)];
+        const T n_x = ldo(pl(x, SXY, kc(Kn + 1)), so);
+        const T n_u1a = ldo(pl(u1, FXY, kc(Kn)), fo), n_u1b = ldo(pl(u1, FXY, kc(Kn)) + 1, fo);
+        const T n_u2a = ldo(pl(u2, YXY, kc(Kn)), so), n_u2b = ldo(pl(u2, YXY, kc(Kn)) + o.yx, so);
+        const T n_u3 = ldo(pl(u3, SXY, fc(Kn + 1)), so);
+        const T n_h = ldo(pl(h, SXY, kc(Kn)), so), n_r = ldo(pl(rhr, SXY, kc(Kn)), so);
         __syncthreads();
         // h and rhr of the output plane Q = K-2, needed in stage D: loaded
         // here (cache hits) rather than carried in registers for two planes
@@ -572,8 +616,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         const bool D_on = do_d && Q >= k0 && Q <= k1;
         T hQ = T(1), rQ = T(1);
         if (D_on) {
-            hQ = h[bs + SXY * Q];
-            rQ = rhr[bs + SXY * Q];
+            hQ = ldo(pl(h, SXY, Q), so);
+            rQ = ldo(pl(rhr, SXY, Q), so);
         }
         // ---- C: cp, cn on plane P ------------------------------------------
         if (BC_on) {
@@ -630,15 +674,15 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             }
 #undef CP
 #undef CN
-            const int c = bs + SXY * Q;
+            T* const xoq = pl(xo, SXY, Q);
             const T r = rQ;
-            xo[c] = r * (xaq1 / r - (F1p - F1 + F2p - F2 + F3p - F3) / hQ);
+            sto(xoq, so, r * (xaq1 / r - (F1p - F1 + F2p - F2 + F3p - F3) / hQ));
             if (gi == 1 || gi == np || gj == 1 || gj == mp) {
-                const T xe = x[c];
-                if (gi == 1) xo[c - 1] = xe;
-                if (gi == np) xo[c + 1] = xe;
-                if (gj == 1) xo[c - o.sx] = xe;
-                if (gj == mp) xo[c + o.sx] = xe;
+                const T xe = ldo(pl(x, SXY, Q), so);
+                if (gi == 1) sto(xoq - 1, so, xe);
+                if (gi == np) sto(xoq + 1, so, xe);
+                if (gj == 1) sto(xoq - o.sx, so, xe);
+                if (gj == mp) sto(xoq + o.sx, so, xe);
             }
         }
 #undef XA
@@ -902,9 +946,13 @@ template <typename T>
 bool Fused<T>::supports(const Geo& g, int variant, int iters, int nonos) {
     if (variant != 1 || iters != 2 || nonos != 1) return false;
     if (g.px || g.py || g.pz) return false;
-    // 32-bit offsets inside the kernel
+    // 32-bit element offsets inside the kernel, and 32-bit byte offsets
+    // inside one plane of every array
     const long long biggest = len_of<FACEX>(g) > len_of<FACEY>(g) ? len_of<FACEX>(g) : len_of<FACEY>(g);
-    return biggest < (1ll << 31) - 1 && len_of<FACEZ>(g) < (1ll << 31) - 1;
+    const Off o = make_off(g);
+    const long long plane = std::max(std::max((long long)o.sxy, (long long)o.fxy), (long long)o.yxy);
+    return biggest < (1ll << 31) - 1 && len_of<FACEZ>(g) < (1ll << 31) - 1 &&
+           plane * (long long)sizeof(T) < (1ll << 32);
 }
 
 template <typename T>
