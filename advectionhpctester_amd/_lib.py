@@ -10,7 +10,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmpdata_hip.so")
+# MPDATA_HIP_LIB names a library built elsewhere (same C ABI)
+LIB_PATH = os.environ.get("MPDATA_HIP_LIB") or os.path.join(HERE, "libmpdata_hip.so")
 
 MPDATA_OK, MPDATA_EINVAL, MPDATA_EHIP, MPDATA_ENOMEM, MPDATA_EUNSUPPORTED = 0, 1, 2, 3, 4
 UPWIND, GAUGE, STANDARD = 0, 1, 2

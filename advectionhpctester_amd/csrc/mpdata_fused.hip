@@ -62,6 +62,22 @@ template <typename T>
 __device__ __forceinline__ void sto(T* p, uint32_t off, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off) = v;
 }
+
+// A register copy made here rather than where the loop's phi nodes would
+// put it (the end of the iteration).  Shifting a register queue with these
+// before the new plane is loaded frees the slot the load writes, so the
+// load lands in the register the next iteration reads and no end-of-step
+// copy has to wait for it.
+__device__ __forceinline__ double qcopy(double v) {
+    double r;
+    asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ float qcopy(float v) {
+    float r;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
 template <typename T, int TX, int TY>
@@ -367,14 +383,16 @@ struct FusedLds {
     T sv2[2][RY][RX];
     T scp[3][RY][RX];
     T scn[3][RY][RX];
+    T sf1[RY][RX];  // limited flux through the left x-face, plane K-2
+    T sf2[RY][RX];  // limited flux through the lower y-face, plane K-2
 };
 
-// One work item: output tile `tile` (ntx tiles per row) over the planes of
-// k-chunk `chunk` (clen planes).  Every argument is uniform over the
-// workgroup, so all threads pass the same barriers.
+// One work item: output tile `tile` (ntx tiles per row) over the output
+// planes [k0, k1].  Every argument is uniform over the workgroup, so all
+// threads pass the same barriers.
 template <typename T, int RX, int RY>
-__device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, int clen, int tile,
-                                           int chunk, const T* __restrict__ x, const T* __restrict__ u1,
+__device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, int tile, int k0,
+                                           int k1, const T* __restrict__ x, const T* __restrict__ u1,
                                            const T* __restrict__ u2, const T* __restrict__ u3,
                                            const T* __restrict__ rhr, const T* __restrict__ h,
                                            const T* __restrict__ bcx, const T* __restrict__ bcy,
@@ -389,6 +407,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     auto& sv2 = L.sv2;
     auto& scp = L.scp;
     auto& scn = L.scn;
+    auto& sf1 = L.sf1;
+    auto& sf2 = L.sf2;
     const int np = g.np, mp = g.mp, lp = g.lp;
     const int t = threadIdx.x;
     const int pi = t % RX, pj = t / RX;  // region index
@@ -418,7 +438,6 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
 
 
     const int i0 = 1 + (tile % ntx) * TX, j0 = 1 + (tile / ntx) * TY;
-    const int k0 = 1 + chunk * clen, k1 = min(lp, k0 + clen - 1);
     const int gi = i0 + li, gj = j0 + lj;
     const bool in_dom = gi >= 1 && gi <= np && gj >= 1 && gj <= mp;
     // roles of this position
@@ -426,6 +445,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     const bool do_v2 = in_dom && gj >= 2 && li >= -1 && li <= TX && lj >= -1;
     const bool do_c = in_dom && li >= -1 && li <= TX && lj >= -1 && lj <= TY;
     const bool do_d = in_dom && li >= 0 && li < TX && lj >= 0 && lj < TY;
+    // left/lower face fluxes: every x-face and y-face of a tile cell
+    const bool do_f = li >= 0 && li <= TX && lj >= 0 && lj <= TY;
     // upwind source cell: itself in the domain, the mirror image on the
     // one-cell halo ring (cp_scnd_last_to_halo_xy_fulld), clamped beyond
     const int ai = clampi(mirror(gi, np), 1, np), aj = clampi(mirror(gj, mp), 1, mp);
@@ -449,9 +470,11 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     // register queues of the source column; during step K they hold
     //   xq1..3 = x(K-1..K+1), u1a0/1 = u1(i,K-1..K), u1b0/1 = u1(i+1,K-1..K),
     //   u2a/u2b likewise at j and j+1, u3q0..2 = faces K-1..K+1,
-    //   hq1..2 = h(K-1..K), rq2 = rhr(K), xaq1..2 = xant(K-2..K-1),
-    //   s1q/s2q = pair sums of planes K-2..K-1, v3q1..2 = V3 faces K-2..K-1,
-    //   xmx0/xmn0 = bounds of x for the limiter on plane K-1
+    //   hq1..2 = h(K-1..K), rq2 = rhr(K), v3q1..2 = V3 faces K-2..K-1,
+    //   xmx0/xmn0 = bounds of x for the limiter on plane K-1.
+    // The column's own xant of planes K-2..K comes from sxa: the slot of
+    // an own position is rewritten only by this thread, later in program
+    // order.
     T xq1 = ldo(pl(x, SXY, kc(Ks - 1)), so), xq2 = ldo(pl(x, SXY, kc(Ks)), so);
     T xq3 = ldo(pl(x, SXY, kc(Ks + 1)), so);
     T u1a0 = ldo(pl(u1, FXY, kc(Ks - 1)), fo), u1a1 = ldo(pl(u1, FXY, kc(Ks)), fo);
@@ -462,10 +485,9 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     T u3q2 = ldo(pl(u3, SXY, fc(Ks + 1)), so);
     T hq1 = ldo(pl(h, SXY, kc(Ks - 1)), so), hq2 = ldo(pl(h, SXY, kc(Ks)), so);
     T rq2 = ldo(pl(rhr, SXY, kc(Ks)), so);
-    T xaq1 = T(0), xaq2 = T(0);
-    T s1q1 = T(0), s1q2 = T(0), s2q1 = T(0), s2q2 = T(0);
     T v3q1 = T(0), v3q2 = T(0);
     T xmx0 = T(0), xmn0 = T(0), xmx1 = T(0), xmn1 = T(0);  // x in-plane max/min, planes K-1, K
+    T f3c = T(0);  // limited flux through the upper z-face of the last output plane
 
     // x of the first plane into LDS; the barrier also orders this item
     // after every LDS read of the previous one
@@ -489,27 +511,26 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             if (!in_dom) xown = ld(own, K + 1);
         }
         // neighbour-column inputs of V1/V2 on plane P=K-1, loaded before
-        // the barriers so that their latency overlaps stage A
+        // stage A so that their latency overlaps it.  They are loaded by
+        // every lane on every plane (the plane clamped into the domain; the
+        // source column is in the domain, so its i-1 and j-1 neighbours are
+        // in the arrays), which costs nothing in a wave that loads anyway,
+        // and used only where V1/V2 are formed.
         const int P = K - 1;
         const bool BC_on = P >= max(1, k0 - 1) && P <= min(lp, k1 + 1);
-        const bool zt = !(P == 1 || P == lp);
-        T nv1_h = T(0), nv1_u2a = T(0), nv1_u2b = T(0), nv1_u3a = T(0), nv1_u3b = T(0);
-        T nv2_h = T(0), nv2_u1a = T(0), nv2_u1b = T(0), nv2_u3a = T(0), nv2_u3b = T(0);
-        if (BC_on && do_v1) {  // column (gi-1, gj)
-            nv1_h = ldo(pl(h, SXY, P) - 1, so);
-            nv1_u2a = ldo(pl(u2, YXY, P) - 1, so);
-            nv1_u2b = ldo(pl(u2, YXY, P) - 1 + o.yx, so);
-            if (zt) { nv1_u3a = ldo(pl(u3, SXY, P) - 1, so); nv1_u3b = ldo(pl(u3, SXY, P + 1) - 1, so); }
-        }
-        if (BC_on && do_v2) {  // column (gi, gj-1)
-            nv2_h = ldo(pl(h, SXY, P) - o.sx, so);
-            nv2_u1a = ldo(pl(u1, FXY, P) - o.fx, fo);
-            nv2_u1b = ldo(pl(u1, FXY, P) - o.fx + 1, fo);
-            if (zt) {
-                nv2_u3a = ldo(pl(u3, SXY, P) - o.sx, so);
-                nv2_u3b = ldo(pl(u3, SXY, P + 1) - o.sx, so);
-            }
-        }
+        const int Pc = kc(P);
+        // column (gi-1, gj)
+        const T nv1_h = ldo(pl(h, SXY, Pc) - 1, so);
+        const T nv1_u2a = ldo(pl(u2, YXY, Pc) - 1, so);
+        const T nv1_u2b = ldo(pl(u2, YXY, Pc) - 1 + o.yx, so);
+        const T nv1_u3a = ldo(pl(u3, SXY, Pc) - 1, so);
+        const T nv1_u3b = ldo(pl(u3, SXY, Pc + 1) - 1, so);
+        // column (gi, gj-1)
+        const T nv2_h = ldo(pl(h, SXY, Pc) - o.sx, so);
+        const T nv2_u1a = ldo(pl(u1, FXY, Pc) - o.fx, fo);
+        const T nv2_u1b = ldo(pl(u1, FXY, Pc) - o.fx + 1, fo);
+        const T nv2_u3a = ldo(pl(u3, SXY, Pc) - o.sx, so);
+        const T nv2_u3b = ldo(pl(u3, SXY, Pc + 1) - o.sx, so);
 
         // ---- A: xant on plane K ------------------------------------------
         T xa_new = T(0);
@@ -554,6 +575,10 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         }
         __syncthreads();
 #define XA(pl, a, b) sxa[(pl) % 3][(b) + 2][(a) + 2]
+#define CP(pl, a, b) scp[(pl) % 3][(b) + 2][(a) + 2]
+#define CN(pl, a, b) scn[(pl) % 3][(b) + 2][(a) + 2]
+        const int Q = K - 2;
+        const bool Dq = Q >= k0 && Q <= k1;
         // ---- B: pair sums, V1/V2 on plane P=K-1, V3 on face K --------------
         T s1n = T(0), s2n = T(0), v3n = T(0);
         if (A_on) {
@@ -568,7 +593,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                        s2 * rat4(XA(K - 1, li, lj - 1), XA(K, li, lj - 1), XA(K - 1, li, lj + 1),
                                  XA(K, li, lj + 1));
                 T a = u3q1;
-                v3n = vdyf(xaq2, xa_new, a, hmz) - T(0.125) * a * hmz * (cr);
+                v3n = vdyf(XA(K - 1, li, lj), xa_new, a, hmz) - T(0.125) * a * hmz * (cr);
             }
         }
         if (BC_on) {
@@ -581,7 +606,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                                  XA(P, li, lj + 1));
                 if (zterms) {
                     T s3 = nv1_u3a + nv1_u3b + u3q1 + u3q0;
-                    cr = cr + s3 * ((s1n - s1q1) * T(0.25));
+                    cr = cr + s3 * ((s1n - (XA(K - 2, li, lj) + XA(K - 2, li - 1, lj))) * T(0.25));
                 }
                 T a = u1a0;
                 val1 = vdyf(XA(P, li - 1, lj), XA(P, li, lj), a, hmx) - T(0.125) * a * hmx * (cr);
@@ -593,7 +618,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                                  XA(P, li + 1, lj));
                 if (zterms) {
                     T s3 = nv2_u3a + u3q0 + u3q1 + nv2_u3b;
-                    cr = cr + s3 * ((s2n - s2q1) * T(0.25));
+                    cr = cr + s3 * ((s2n - (XA(K - 2, li, lj) + XA(K - 2, li, lj - 1))) * T(0.25));
                 }
                 T a = u2a0;
                 val2 = vdyf(XA(P, li, lj - 1), XA(P, li, lj), a, hmy) - T(0.125) * a * hmy * (cr);
@@ -601,35 +626,54 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             sv1[P & 1][pj][pi] = val1;
             sv2[P & 1][pj][pi] = val2;
         }
-        // prefetch the source-column inputs of plane K+1 (consumed at the
-        // end of this step)
-        const int Kn = K + 1;
-        const T n_x = ldo(pl(x, SXY, kc(Kn + 1)), so);
-        const T n_u1a = ldo(pl(u1, FXY, kc(Kn)), fo), n_u1b = ldo(pl(u1, FXY, kc(Kn)) + 1, fo);
-        const T n_u2a = ldo(pl(u2, YXY, kc(Kn)), so), n_u2b = ldo(pl(u2, YXY, kc(Kn)) + o.yx, so);
-        const T n_u3 = ldo(pl(u3, SXY, fc(Kn + 1)), so);
-        const T n_h = ldo(pl(h, SXY, kc(Kn)), so), n_r = ldo(pl(rhr, SXY, kc(Kn)), so);
-        __syncthreads();
-        // h and rhr of the output plane Q = K-2, needed in stage D: loaded
-        // here (cache hits) rather than carried in registers for two planes
-        const int Q = K - 2;
-        const bool D_on = do_d && Q >= k0 && Q <= k1;
-        T hQ = T(1), rQ = T(1);
-        if (D_on) {
-            hQ = ldo(pl(h, SXY, Q), so);
-            rQ = ldo(pl(rhr, SXY, Q), so);
+        // ---- limited fluxes through the left x-face and lower y-face of this
+        //      position on plane Q (V and cp/cn of Q are from the previous
+        //      step); stage D reads them for both faces a cell shares, so
+        //      each face flux is formed once
+        if (Dq && do_f) {
+            sf1[pj][pi] = gflux(sv1[Q & 1][pj][pi], CP(Q, li, lj), CN(Q, li - 1, lj),
+                                CP(Q, li - 1, lj), CN(Q, li, lj));
+            sf2[pj][pi] = gflux(sv2[Q & 1][pj][pi], CP(Q, li, lj), CN(Q, li, lj - 1),
+                                CP(Q, li, lj - 1), CN(Q, li, lj));
         }
+        // The loaded source-column queues are not read after this point:
+        // rotate them to plane K+1 and load the new planes into the freed
+        // slots.  Each load lands in the register the next step reads, so
+        // no copy at the end of the step waits for it; its latency is
+        // covered by C, D and the next step's barrier.
+        {
+            const int Kn = K + 1;
+            xq1 = qcopy(xq2); xq2 = qcopy(xq3);
+            u1a0 = qcopy(u1a1); u1b0 = qcopy(u1b1); u2a0 = qcopy(u2a1); u2b0 = qcopy(u2b1);
+            u3q0 = qcopy(u3q1); u3q1 = qcopy(u3q2);
+            hq1 = qcopy(hq2);
+            xq3 = ldo(pl(x, SXY, kc(Kn + 1)), so);
+            u1a1 = ldo(pl(u1, FXY, kc(Kn)), fo);
+            u1b1 = ldo(pl(u1, FXY, kc(Kn)) + 1, fo);
+            u2a1 = ldo(pl(u2, YXY, kc(Kn)), so);
+            u2b1 = ldo(pl(u2, YXY, kc(Kn)) + o.yx, so);
+            u3q2 = ldo(pl(u3, SXY, fc(Kn + 1)), so);
+            hq2 = ldo(pl(h, SXY, kc(Kn)), so);
+            rq2 = ldo(pl(rhr, SXY, kc(Kn)), so);
+        }
+        __syncthreads();
+        // h of plane P for C, h and rhr of the output plane Q = K-2 for D:
+        // loaded here (cache hits) rather than carried in registers
+        const bool D_on = do_d && Dq;
+        const T hP = ldo(pl(h, SXY, kc(P)), so);
+        const T hQ = ldo(pl(h, SXY, kc(Q)), so), rQ = ldo(pl(rhr, SXY, kc(Q)), so);
         // ---- C: cp, cn on plane P ------------------------------------------
         if (BC_on) {
             const bool gnd = P == 1, sky = P == lp;
             T rp = T(0), rn = T(0);
             if (do_c) {
                 T mxo = xmx0, mno = xmn0;
+                const T xaP = XA(P, li, lj);
                 T mx = XA(P, li - 1, lj), mn = mx;
 #define ACC(v, lo, hi) { T t_ = (v); lo = min2(lo, t_); hi = max2(hi, t_); }
-                ACC(xaq2, mn, mx) ACC(XA(P, li + 1, lj), mn, mx)
+                ACC(xaP, mn, mx) ACC(XA(P, li + 1, lj), mn, mx)
                 ACC(XA(P, li, lj - 1), mn, mx) ACC(XA(P, li, lj + 1), mn, mx)
-                if (!gnd) ACC(xaq1, mn, mx)
+                if (!gnd) ACC(XA(P - 1, li, lj), mn, mx)
                 if (!sky) ACC(xa_new, mn, mx)
 #undef ACC
                 mx = max2(mx, mxo);
@@ -640,9 +684,9 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 if (gnd) { a3p = v3n; a3 = -a3p; }
                 else if (sky) { a3 = v3q2; a3p = -a3; }
                 else { a3p = v3n; a3 = v3q2; }
-                T hc = hq1;
-                rp = (mx - xaq2) * hc / (pn(a1p) + pp(a1) + pn(a2p) + pp(a2) + pn(a3p) + pp(a3) + ep);
-                rn = (xaq2 - mn) * hc / (pp(a1p) + pn(a1) + pp(a2p) + pn(a2) + pp(a3p) + pn(a3) + ep);
+                T hc = hP;
+                rp = (mx - xaP) * hc / (pn(a1p) + pp(a1) + pn(a2p) + pp(a2) + pn(a3p) + pp(a3) + ep);
+                rn = (xaP - mn) * hc / (pp(a1p) + pn(a1) + pp(a2p) + pn(a2) + pp(a3p) + pn(a3) + ep);
             }
             scp[P % 3][pj][pi] = rp;
             scn[P % 3][pj][pi] = rn;
@@ -651,32 +695,26 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         //      written in the previous step; own cp/cn of Q+1 just now) ----
         if (D_on) {
             const bool gnd = Q == 1, sky = Q == lp;
-#define CP(pl, a, b) scp[(pl) % 3][(b) + 2][(a) + 2]
-#define CN(pl, a, b) scn[(pl) % 3][(b) + 2][(a) + 2]
-            const T v1c = sv1[Q & 1][pj][pi], v1p = sv1[Q & 1][pj][pi + 1];
-            const T v2c = sv2[Q & 1][pj][pi], v2p = sv2[Q & 1][pj + 1][pi];
-            T F1 = gflux(v1c, CP(Q, li, lj), CN(Q, li - 1, lj), CP(Q, li - 1, lj), CN(Q, li, lj));
-            T F1p = gflux(v1p, CP(Q, li + 1, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q, li + 1, lj));
-            T F2 = gflux(v2c, CP(Q, li, lj), CN(Q, li, lj - 1), CP(Q, li, lj - 1), CN(Q, li, lj));
-            T F2p = gflux(v2p, CP(Q, li, lj + 1), CN(Q, li, lj), CP(Q, li, lj), CN(Q, li, lj + 1));
+            // x/y faces from stage B of this step; the lower z-face flux is
+            // the upper one of plane Q-1 (the same expression), except on
+            // the first plane of the range
+            const T F1 = sf1[pj][pi], F1p = sf1[pj][pi + 1];
+            const T F2 = sf2[pj][pi], F2p = sf2[pj + 1][pi];
             T F3, F3p;
             if (gnd) {
                 const T v = v3q2;
                 F3 = gflux(-v, CP(Q, li, lj), CN(Q + 1, li, lj), CP(Q + 1, li, lj), CN(Q, li, lj));
                 F3p = gflux(v, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
-            } else if (sky) {
-                const T v = v3q1;
-                F3 = gflux(v, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
-                F3p = gflux(-v, CP(Q - 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q - 1, li, lj));
             } else {
-                F3 = gflux(v3q1, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
-                F3p = gflux(v3q2, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
+                if (Q > k0) F3 = f3c;
+                else F3 = gflux(v3q1, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
+                if (sky) F3p = gflux(-v3q1, CP(Q - 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q - 1, li, lj));
+                else F3p = gflux(v3q2, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
             }
-#undef CP
-#undef CN
+            f3c = F3p;
             T* const xoq = pl(xo, SXY, Q);
             const T r = rQ;
-            sto(xoq, so, r * (xaq1 / r - (F1p - F1 + F2p - F2 + F3p - F3) / hQ));
+            sto(xoq, so, r * (XA(Q, li, lj) / r - (F1p - F1 + F2p - F2 + F3p - F3) / hQ));
             if (gi == 1 || gi == np || gj == 1 || gj == mp) {
                 const T xe = ldo(pl(x, SXY, Q), so);
                 if (gi == 1) sto(xoq - 1, so, xe);
@@ -686,16 +724,9 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             }
         }
 #undef XA
-        // ---- rotate the queues to plane K+1 ----------------------------------
-        xq1 = xq2; xq2 = xq3; xq3 = n_x;
-        u1a0 = u1a1; u1a1 = n_u1a; u1b0 = u1b1; u1b1 = n_u1b;
-        u2a0 = u2a1; u2a1 = n_u2a; u2b0 = u2b1; u2b1 = n_u2b;
-        u3q0 = u3q1; u3q1 = u3q2; u3q2 = n_u3;
-        hq1 = hq2; hq2 = n_h;
-        rq2 = n_r;
-        xaq1 = xaq2; xaq2 = xa_new;
-        s1q1 = s1q2; s1q2 = s1n;
-        s2q1 = s2q2; s2q2 = s2n;
+#undef CP
+#undef CN
+        // ---- rotate the computed queues to plane K+1 --------------------------
         v3q1 = v3q2; v3q2 = v3n;
         xmx0 = xmx1; xmn0 = xmn1;
     }
@@ -710,24 +741,25 @@ k_fused_gauge2(Geo g, Off o, int ntx, int nty, int nchunk, int clen,
     __shared__ FusedLds<T, RX, RY> L;
     const int ntile = ntx * nty;
     // Work-item order: workgroups are dealt to the XCDs round-robin
-    // (blockIdx % NXCD), each with its own L2.  Item w of workgroup b maps so
-    // that the workgroups of one XCD process a contiguous run of tiles
-    // (neighbours share halo columns in that L2) and all XCDs advance through
-    // the k-chunks together.
+    // (blockIdx % NXCD), each with its own L2.  The workgroups of one XCD
+    // take a contiguous run of tiles, item by item in rounds, so that at any
+    // time they march neighbouring tiles through the same k-chunk and share
+    // the halo columns in that L2.  (A split into equal contiguous plane
+    // ranges per workgroup balances the tail better but loses this
+    // alignment and measured slower.)
     const int nxcd = gridDim.x >= NXCD ? NXCD : 1;
     const int per_xcd = gridDim.x / nxcd;                 // workgroups per XCD
     const int xcd = blockIdx.x % nxcd, slot = blockIdx.x / nxcd;
     const int nwg = per_xcd * nxcd;                       // workgroups that take items
     if (blockIdx.x >= nwg) return;
+    const int tiles_per_xcd = (ntile + nxcd - 1) / nxcd;
     for (int r = slot; ; r += per_xcd) {
-        // r-th item of this XCD; items of an XCD are tile-major within a
-        // k-chunk, and the tile range of the chunk is split evenly over XCDs
-        const int tiles_per_xcd = (ntile + nxcd - 1) / nxcd;
         const int chunk = r / tiles_per_xcd, tix = r - chunk * tiles_per_xcd;
         if (chunk >= nchunk) break;
         const int tile = xcd * tiles_per_xcd + tix;
         if (tile >= ntile) continue;
-        fused_item<T, RX, RY>(g, o, ntx, clen, tile, chunk, x, u1, u2, u3, rhr, h, bcx, bcy, xo, L);
+        const int k0 = 1 + chunk * clen, k1 = min(g.lp, k0 + clen - 1);
+        fused_item<T, RX, RY>(g, o, ntx, tile, k0, k1, x, u1, u2, u3, rhr, h, bcx, bcy, xo, L);
     }
 }
 
@@ -785,7 +817,7 @@ __global__ void k_copy_untouched_halo(Geo g, const T* __restrict__ x, T* __restr
 constexpr int FTX = 32, FTY = 8;
 
 // Kernel generation (1 = per-region strided loops, 2 = fixed positions),
-// tile configuration of generation 2 (0 = 32x16 lanes at 4 waves per SIMD,
+// tile configuration of generation 2 (0 = 32x32 lanes at 4 waves per SIMD,
 // the fastest at 512x512x256; 1-4 the alternatives in the switch below) and
 // k-chunk length; MPDATA_FUSED_GEN / _CFG / _CHUNK override them for tuning
 // runs.
@@ -797,6 +829,7 @@ int fused_config() {
     const char* e = getenv("MPDATA_FUSED_CFG");
     return e ? atoi(e) : 0;
 }
+
 int fused_chunk_len(int lp) {
     const char* e = getenv("MPDATA_FUSED_CHUNK");
     int c = e ? atoi(e) : 64;
@@ -822,8 +855,8 @@ hipError_t launch2(const Geo& g, const Off& o, const T* x, T* xout, const T* u1,
                                                          RX * RY, 0);
     if (e != hipSuccess) return e;
     const int grid = std::max(1, std::min(nitem, ncu * std::max(per_cu, 1)));
-    k_fused_gauge2<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>(g, o, ntx, nty, nchunk, clen, x, u1, u2,
-                                                             u3, rhr, h, bcx, bcy, xout);
+    k_fused_gauge2<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>(g, o, ntx, nty, nchunk, clen, x, u1,
+                                                             u2, u3, rhr, h, bcx, bcy, xout);
     return hipGetLastError();
 }
 
@@ -863,7 +896,8 @@ k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem) {
         const Member<T>& m = mem[lo];
         const int l = w - m.item0;
         const int chunk = l / m.ntile, tile = l - chunk * m.ntile;
-        fused_item<T, RX, RY>(m.g, m.o, m.ntx, m.clen, tile, chunk, m.x, m.u1, m.u2, m.u3, m.rhr,
+        const int k0 = 1 + chunk * m.clen, k1 = min(m.g.lp, k0 + m.clen - 1);
+        fused_item<T, RX, RY>(m.g, m.o, m.ntx, tile, k0, k1, m.x, m.u1, m.u2, m.u3, m.rhr,
                               m.h, m.bcx, m.bcy, m.xo, L);
     }
 }
@@ -972,11 +1006,11 @@ hipError_t Fused<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const 
     }
     hipError_t e = hipSuccess;
     switch (fused_config()) {
-        case 1: e = launch2<T, 32, 16, 2>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
-        case 2: e = launch2<T, 32, 32, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+        case 1: e = launch2<T, 32, 16, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+        case 2: e = launch2<T, 32, 16, 2>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
         case 3: e = launch2<T, 64, 16, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
         case 4: e = launch2<T, 16, 32, 2>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
-        default: e = launch2<T, 32, 16, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+        default: e = launch2<T, 32, 32, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
     }
     if (e != hipSuccess) return e;
     k_copy_untouched_halo<T><<<256, 256, 0, s>>>(g, x, xout);
