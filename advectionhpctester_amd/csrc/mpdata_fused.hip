@@ -78,6 +78,18 @@ __device__ __forceinline__ float qcopy(float v) {
     asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
     return r;
 }
+
+// The value held by lane l-1 of the wave (DPP wave_shr:1; lane 0 gets 0).
+// Convergent: called with every lane of the wave active.
+__device__ __forceinline__ double from_left_lane(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ __forceinline__ float from_left_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
 template <typename T, int TX, int TY>
@@ -400,6 +412,9 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                                            FusedLds<T, RX, RY>& L) {
     constexpr int NT = RX * RY, TX = RX - 4, TY = RY - 4;
     constexpr int SW = RX + 2, SH = RY + 2, NRING = 2 * SW + 2 * (SH - 2);
+    // lane t-1 holds region position (pi-1, pj) for every pi >= 1, in the
+    // same wave, and the workgroup is whole waves
+    static_assert((64 % RX == 0 || RX % 64 == 0) && NT % 64 == 0, "row layout of the waves");
     static_assert(NRING <= NT, "x ring must fit one extra slot per thread");
     auto& sx = L.sx;
     auto& sxa = L.sxa;
@@ -510,21 +525,23 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             if (t < NRING) xr = ld(ring, K + 1);
             if (!in_dom) xown = ld(own, K + 1);
         }
-        // neighbour-column inputs of V1/V2 on plane P=K-1, loaded before
-        // stage A so that their latency overlaps it.  They are loaded by
-        // every lane on every plane (the plane clamped into the domain; the
-        // source column is in the domain, so its i-1 and j-1 neighbours are
-        // in the arrays), which costs nothing in a wave that loads anyway,
-        // and used only where V1/V2 are formed.
+        // neighbour-column inputs of V1/V2 on plane P=K-1.  Column
+        // (gi-1, gj) is the source column of the lane to the left wherever
+        // V1 is formed (gi >= 2 in the domain, so the left position is in
+        // the domain too), and its queues hold h, u2 and u3 of plane P:
+        // taken across lanes.  Column (gi, gj-1) is loaded, before stage A
+        // so that the latency overlaps it, by every lane on every plane (the
+        // plane clamped into the domain; the source column is in the
+        // domain, so its j-1 neighbour is in the arrays), which costs
+        // nothing in a wave that loads anyway; used only where V2 is formed.
         const int P = K - 1;
         const bool BC_on = P >= max(1, k0 - 1) && P <= min(lp, k1 + 1);
         const int Pc = kc(P);
-        // column (gi-1, gj)
-        const T nv1_h = ldo(pl(h, SXY, Pc) - 1, so);
-        const T nv1_u2a = ldo(pl(u2, YXY, Pc) - 1, so);
-        const T nv1_u2b = ldo(pl(u2, YXY, Pc) - 1 + o.yx, so);
-        const T nv1_u3a = ldo(pl(u3, SXY, Pc) - 1, so);
-        const T nv1_u3b = ldo(pl(u3, SXY, Pc + 1) - 1, so);
+        const T nv1_h = from_left_lane(hq1);
+        const T nv1_u2a = from_left_lane(u2a0);
+        const T nv1_u2b = from_left_lane(u2b0);
+        const T nv1_u3a = from_left_lane(u3q0);
+        const T nv1_u3b = from_left_lane(u3q1);
         // column (gi, gj-1)
         const T nv2_h = ldo(pl(h, SXY, Pc) - o.sx, so);
         const T nv2_u1a = ldo(pl(u1, FXY, Pc) - o.fx, fo);
