@@ -837,31 +837,63 @@ constexpr int FTX = 32, FTY = 8;
 // tile configuration of generation 2 (0 = 32x32 lanes at 4 waves per SIMD,
 // the fastest at 512x512x256; 1-4 the alternatives in the switch below) and
 // k-chunk length; MPDATA_FUSED_GEN / _CFG / _CHUNK override them for tuning
-// runs.
+// runs (-1: not set).
 int fused_generation() {
     const char* e = getenv("MPDATA_FUSED_GEN");
     return e && atoi(e) == 1 ? 1 : 2;
 }
 int fused_config() {
     const char* e = getenv("MPDATA_FUSED_CFG");
-    return e ? atoi(e) : 0;
+    return e ? atoi(e) : -1;
+}
+int fused_chunk_env() {
+    const char* e = getenv("MPDATA_FUSED_CHUNK");
+    const int c = e ? atoi(e) : -1;
+    return c >= 1 ? c : -1;
 }
 
-int fused_chunk_len(int lp) {
-    const char* e = getenv("MPDATA_FUSED_CHUNK");
-    int c = e ? atoi(e) : 64;
-    if (c < 1) c = 64;
-    return c > lp ? lp : c;
+template <int RX, int RY>
+long long items_of(const Geo& g, int clen) {
+    const long long ntx = (g.np + RX - 5) / (RX - 4), nty = (g.mp + RY - 5) / (RY - 4);
+    return ntx * nty * ((g.lp + clen - 1) / clen);
+}
+
+// Tiling and k-chunk of a single-domain step.  32x32 tiles recompute the
+// fewest ring cells but give one workgroup per CU; a domain too small to
+// give every CU two of them takes 32x16 tiles (two workgroups per CU) and
+// the longest k-chunk (64, 32 or 16 planes) that gives every workgroup
+// slot two items, or 16.
+void choose_tiling(const Geo& g, int& cfg, int& clen) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu < 1)
+        ncu = 256;
+    const int env_cfg = fused_config(), env_clen = fused_chunk_env();
+    if (env_cfg >= 0) {
+        cfg = env_cfg;
+        clen = env_clen > 0 ? env_clen : 64;
+    } else if (items_of<32, 32>(g, env_clen > 0 ? env_clen : 64) >= 2LL * ncu) {
+        cfg = 0;
+        clen = env_clen > 0 ? env_clen : 64;
+    } else {
+        cfg = 1;
+        clen = env_clen > 0 ? env_clen : 16;
+        if (env_clen <= 0)
+            for (int c : {64, 32}) {
+                if (items_of<32, 16>(g, c) >= 4LL * ncu) { clen = c; break; }
+            }
+    }
+    if (clen > g.lp) clen = g.lp;
 }
 
 // Persistent launch of the fixed-position kernel: as many workgroups as fit
 // on the device at once, each looping over (tile, k-chunk) work items.
 template <typename T, int RX, int RY, int MINB>
-hipError_t launch2(const Geo& g, const Off& o, const T* x, T* xout, const T* u1, const T* u2,
-                   const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+hipError_t launch2(const Geo& g, const Off& o, int clen, const T* x, T* xout, const T* u1,
+                   const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
                    hipStream_t s) {
     const int ntx = (g.np + RX - 5) / (RX - 4), nty = (g.mp + RY - 5) / (RY - 4);
-    const int clen = fused_chunk_len(g.lp);
     const int nchunk = (g.lp + clen - 1) / clen;
     const int nitem = ntx * nty * nchunk;
     int dev = 0, ncu = 0, per_cu = 0;
@@ -1022,12 +1054,14 @@ hipError_t Fused<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const 
         return hipGetLastError();
     }
     hipError_t e = hipSuccess;
-    switch (fused_config()) {
-        case 1: e = launch2<T, 32, 16, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
-        case 2: e = launch2<T, 32, 16, 2>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
-        case 3: e = launch2<T, 64, 16, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
-        case 4: e = launch2<T, 16, 32, 2>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
-        default: e = launch2<T, 32, 32, 4>(g, o, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+    int cfg = 0, cl = 64;
+    choose_tiling(g, cfg, cl);
+    switch (cfg) {
+        case 1: e = launch2<T, 32, 16, 4>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+        case 2: e = launch2<T, 32, 16, 2>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+        case 3: e = launch2<T, 64, 16, 4>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+        case 4: e = launch2<T, 16, 32, 2>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
+        default: e = launch2<T, 32, 32, 4>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
     }
     if (e != hipSuccess) return e;
     k_copy_untouched_halo<T><<<256, 256, 0, s>>>(g, x, xout);
@@ -1044,11 +1078,12 @@ template struct Fused<float>;
     X(1, 32, 16, 4)             \
     X(2, 64, 16, 4)             \
     X(3, 16, 32, 2)             \
-    X(4, 64, 8, 2)
+    X(4, 64, 8, 2)              \
+    X(5, 32, 32, 4)
 
 template <typename T>
 int FusedBatch<T>::nconfigs() {
-    return 5;
+    return 6;
 }
 
 template <typename T>
