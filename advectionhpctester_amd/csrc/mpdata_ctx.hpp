@@ -50,6 +50,13 @@ struct mpdata_ctx {
     std::vector<mpdata::TapeSlot> tape;
     // adjoint work arrays
     void* adj[24] = {};
+    // same-grid multi-field step: output field per batched field, the
+    // device descriptor table and the pointers it was packed for
+    std::vector<void*> mf_out;
+    void* mf_table = nullptr;
+    size_t mf_table_bytes = 0;
+    std::vector<const void*> mf_key;
+    int mf_cfg = -1, mf_clen = 0, mf_nitem = 0;
     long long held = 0;     // bytes of device scratch
     std::vector<void*> owned;
 };

@@ -926,7 +926,7 @@ struct Member {  // device descriptor of one member
 
 template <typename T, int RX, int RY, int MINB>
 __global__ void __launch_bounds__(RX * RY, MINB)
-k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem) {
+k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem, int shared) {
     __shared__ FusedLds<T, RX, RY> L;
     const int nxcd = gridDim.x >= NXCD ? NXCD : 1;
     const int per_xcd = gridDim.x / nxcd;
@@ -936,14 +936,24 @@ k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem) {
     for (int r = slot; r < items_per_xcd; r += per_xcd) {
         const int w = xcd * items_per_xcd + r;
         if (w >= nitem) break;
-        int lo = 0, hi = nmem - 1;  // last member with item0 <= w
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (mem[mid].item0 <= w) lo = mid;
-            else hi = mid - 1;
+        int q, l;
+        if (shared) {
+            // members on one grid (fields sharing the velocity): the items
+            // of one (tile, chunk) for all members are consecutive, so one
+            // XCD marches them together and reads u, h, rhr once into its L2
+            q = w % nmem;
+            l = w / nmem;
+        } else {
+            int lo = 0, hi = nmem - 1;  // last member with item0 <= w
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (mem[mid].item0 <= w) lo = mid;
+                else hi = mid - 1;
+            }
+            q = lo;
+            l = w - mem[q].item0;
         }
-        const Member<T>& m = mem[lo];
-        const int l = w - m.item0;
+        const Member<T>& m = mem[q];
         const int chunk = l / m.ntile, tile = l - chunk * m.ntile;
         const int k0 = 1 + chunk * m.clen, k1 = min(m.g.lp, k0 + m.clen - 1);
         fused_item<T, RX, RY>(m.g, m.o, m.ntx, tile, k0, k1, m.x, m.u1, m.u2, m.u3, m.rhr,
@@ -1010,7 +1020,7 @@ int pack_members(const BatchMember<T>* bm, int n, int clen, Member<T>* out) {
 }
 
 template <typename T, int RX, int RY, int MINB>
-hipError_t launch_batch(const void* table, int n, int nitem, hipStream_t s) {
+hipError_t launch_batch(const void* table, int n, int nitem, int shared, hipStream_t s) {
     int dev = 0, ncu = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
@@ -1019,7 +1029,8 @@ hipError_t launch_batch(const void* table, int n, int nitem, hipStream_t s) {
             &per_cu, k_fused_gauge2_batch<T, RX, RY, MINB>, RX * RY, 0);
     if (e != hipSuccess) return e;
     const int grid = std::max(1, std::min(nitem, ncu * std::max(per_cu, 1)));
-    k_fused_gauge2_batch<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>((const Member<T>*)table, n, nitem);
+    k_fused_gauge2_batch<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>((const Member<T>*)table, n, nitem,
+                                                                   shared);
     return hipGetLastError();
 }
 
@@ -1118,11 +1129,11 @@ int FusedBatch<T>::pack(const BatchMember<T>* m, int n, int cfg, int clen, void*
 
 template <typename T>
 hipError_t FusedBatch<T>::step(const void* table, int n, int nitem, int cfg, long long max_halo,
-                               hipStream_t s) {
+                               hipStream_t s, int shared) {
     hipError_t e = hipErrorInvalidValue;
     switch (cfg) {
 #define LAUNCH(c, rx, ry, mb) \
-    case c: e = launch_batch<T, rx, ry, mb>(table, n, nitem, s); break;
+    case c: e = launch_batch<T, rx, ry, mb>(table, n, nitem, shared, s); break;
         MPDATA_BATCH_CONFIGS(LAUNCH)
 #undef LAUNCH
     }
@@ -1132,6 +1143,28 @@ hipError_t FusedBatch<T>::step(const void* table, int n, int nitem, int cfg, lon
     return hipGetLastError();
 }
 #undef MPDATA_BATCH_CONFIGS
+
+// Batch tiling for n members on one grid: as choose_tiling with n times
+// the work items (configuration numbers of MPDATA_BATCH_CONFIGS).
+template <typename T>
+void FusedBatch<T>::choose_shared(const Geo& g, int n, int& cfg, int& clen) {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu < 1)
+        ncu = 256;
+    if (n * items_of<32, 32>(g, 64) >= 2LL * ncu) {
+        cfg = 5;  // 32x32, 4 waves per SIMD
+        clen = 64;
+    } else {
+        cfg = 1;  // 32x16, 4 waves per SIMD
+        clen = 16;
+        for (int c : {64, 32}) {
+            if (n * items_of<32, 16>(g, c) >= 4LL * ncu) { clen = c; break; }
+        }
+    }
+    if (clen > g.lp) clen = g.lp;
+}
 
 template struct FusedBatch<double>;
 template struct FusedBatch<float>;

@@ -41,9 +41,13 @@ struct FusedBatch {
     // member is outside the fused kernel's scope or an argument is invalid
     static int pack(const BatchMember<T>* m, int n, int cfg, int clen, void* table);
     // one batched step with the device copy of a packed table; max_halo =
-    // the largest number of halo cells of one member
+    // the largest number of halo cells of one member.  shared != 0: all
+    // members have one grid, and the work items of one tile and k-chunk run
+    // together for all members (fields sharing a velocity read it once)
     static hipError_t step(const void* table, int n, int nitem, int cfg, long long max_halo,
-                           hipStream_t s);
+                           hipStream_t s, int shared = 0);
+    // tile configuration and k-chunk for n members on grid g
+    static void choose_shared(const Geo& g, int n, int& cfg, int& clen);
 };
 
 }  // namespace mpdata

@@ -195,6 +195,35 @@ class Domain:
                                             self._p(x, 0, "x"), pu1, pu2, pu3, prhr, ph, pbx, pby,
                                             self._stream()))
 
+    # -- many fields on one grid (CE_advection) ---------------------------------
+    def ce_advection(self, fields, bcx, bcy, roles, u1, u2, u3, rhr, h, alg_choice=0, liner=0,
+                     out=None):
+        """One step of every field in `fields` with one velocity, rhr and h
+        (the reference's CE_advection).  roles[f] = 1 for the fields the
+        reference advects with mpdatm3d_antidiff (u, v, w, tht, pstr), 0 for
+        mpdata3d_antidiff (th, moist fields); alg_choice and liner as
+        mpdata_alg_choice and liner.  In place, or into `out` (a list of
+        arrays) when given."""
+        n = len(fields)
+        if not (len(bcx) == len(bcy) == len(roles) == n) or (out is not None and len(out) != n):
+            raise MpdataError(_lib.MPDATA_EINVAL, "one bcx, bcy, role (and output) per field")
+        pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
+        P = ctypes.c_void_p * n
+        xs = P(*[self._p(x, 0, f"field {f}").value for f, x in enumerate(fields)])
+        bxs, bys = P(), P()
+        for f in range(n):
+            px, py = self._bc(bcx[f], bcy[f])
+            bxs[f], bys[f] = (px.value if px is not None else None), (py.value if py is not None else None)
+        rl = (ctypes.c_int * n)(*[int(r) for r in roles])
+        if out is None:
+            check(self._fn("mpdata_ce_advection")(self._ctx, alg_choice, liner, n, rl, xs, bxs, bys,
+                                                  pu1, pu2, pu3, prhr, ph, self._stream()))
+        else:
+            os_ = P(*[self._p(x, 0, f"out {f}").value for f, x in enumerate(out)])
+            check(self._fn("mpdata_ce_advection_oop")(self._ctx, alg_choice, liner, n, rl, xs, os_,
+                                                      bxs, bys, pu1, pu2, pu3, prhr, ph,
+                                                      self._stream()))
+
     # -- tangent-linear and adjoint steps ----------------------------------------
     def tl_step(self, x, dx, u1, u2, u3, du1, du2, du3, rhr, h, bcx=None, bcy=None,
                 variant=GAUGE, iters=2, nonos=1):

@@ -269,6 +269,43 @@ int mpdata_adj_step_f64(mpdata_ctx *ctx, int slot, const double *u1, const doubl
                         const double *u3, const double *rhr, const double *h, double *xbar,
                         double *ubar1, double *ubar2, double *ubar3, double ufac, void *stream);
 
+/* ---- many fields on one grid: CE_advection -----------------------------------
+ * Replaces CE_advection (advection/src_drivers/advection_CEtimestep_twostep_GPUBC.f90:27-376):
+ * nfields fields (the reference's th, u, v, w, tht, pstr and up to six moist
+ * fields) advanced by one step with one velocity u1,u2,u3 (uadv, vadv,
+ * wadv), one rhr and one h (rhoadv), on the context's grid.  Field f has its
+ * inflow values bcx[f], bcy[f] (x_lr, x_bt) and role[f]: 1 for the fields
+ * the reference passes to mpdatm3d_antidiff (u, v, w, tht, pstr), 0 for
+ * those it passes to mpdata3d_antidiff (th, qv, qc, qr, qi, qs, qg).
+ * alg_choice is mpdata_alg_choice: 1 all gauge, 2 all standard, 0 or > 2
+ * role 1 gauge and role 0 standard (:145-155).  liner != 0: upwind only
+ * (:330-341).  Each field's result equals mpdata_step_* (2 iterations,
+ * limiter on, the variant above) on that field; the gauge fields run as one
+ * launch of the batched fused kernel, reading u, h and rhr once per tile for
+ * all of them.  lupdatemulti and iflip have no effect on one process
+ * without poles and are not arguments.  The in-place form writes through a
+ * context buffer per gauge field and copies back; the out-of-place form
+ * writes x_out directly (no output may overlap an input or another output). */
+#define MPDATA_MAX_FIELDS 64
+int mpdata_ce_advection_f64(mpdata_ctx *ctx, int alg_choice, int liner, int nfields,
+                            const int *role, double *const *x, const double *const *bcx,
+                            const double *const *bcy, const double *u1, const double *u2,
+                            const double *u3, const double *rhr, const double *h, void *stream);
+int mpdata_ce_advection_f32(mpdata_ctx *ctx, int alg_choice, int liner, int nfields,
+                            const int *role, float *const *x, const float *const *bcx,
+                            const float *const *bcy, const float *u1, const float *u2,
+                            const float *u3, const float *rhr, const float *h, void *stream);
+int mpdata_ce_advection_oop_f64(mpdata_ctx *ctx, int alg_choice, int liner, int nfields,
+                                const int *role, const double *const *x_in, double *const *x_out,
+                                const double *const *bcx, const double *const *bcy,
+                                const double *u1, const double *u2, const double *u3,
+                                const double *rhr, const double *h, void *stream);
+int mpdata_ce_advection_oop_f32(mpdata_ctx *ctx, int alg_choice, int liner, int nfields,
+                                const int *role, const float *const *x_in, float *const *x_out,
+                                const float *const *bcx, const float *const *bcy,
+                                const float *u1, const float *u2, const float *u3,
+                                const float *rhr, const float *h, void *stream);
+
 /* ---- ensembles: many domains in one launch ---------------------------------
  * No counterpart in the reference (its batching is many fields on one grid,
  * advection_CEtimestep_twostep_GPUBC.f90).  n members of different shapes

@@ -1,0 +1,155 @@
+"""Many fields on one grid (the reference's CE_advection,
+advection/src_drivers/advection_CEtimestep_twostep_GPUBC.f90:27-376): every
+field's result equals one MPDATA step of that field alone, with the variant
+CE_advection gives it, and equals the oracle's step; in place and out of
+place, gauge fields batched in one launch or stepped one by one."""
+import numpy as np
+import pytest
+import torch
+
+from gpu_helpers import bits_equal, describe_diff, interior, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+# CE_advection's fields: th (role 0), u, v, w, tht, pstr (role 1), qv, qc (role 0)
+ROLES = [0, 1, 1, 1, 1, 1, 0, 0]
+DTYPES = [(torch.float64, np.float64), (torch.float32, np.float32)]
+
+
+@pytest.fixture(scope="module")
+def mp():
+    from advectionhpctester_amd import mpdata
+    return mpdata
+
+
+def _fields(dims, nfields, npdt, seed=11):
+    """Shared velocity, rhr, h; per-field x and inflow values (the standard
+    fields positive, the gauge fields sign-changing)."""
+    from advectionhpctester_amd.testdata import random_case
+    base = random_case(*dims, seed=seed, dtype=npdt)
+    xs, bxs, bys = [], [], []
+    for f in range(nfields):
+        c = random_case(*dims, seed=seed + 100 + f, dtype=npdt, positive=ROLES[f % 8] == 0)
+        xs.append(c["x"])
+        bxs.append(c["bcx"])
+        bys.append(c["bcy"])
+    return base, xs, bxs, bys
+
+
+def _variant(mp, alg_choice, role):
+    if alg_choice == 1:
+        return mp.GAUGE
+    if alg_choice == 2:
+        return mp.STANDARD
+    return mp.GAUGE if role else mp.STANDARD
+
+
+@pytest.mark.parametrize("tdt,npdt", DTYPES)
+@pytest.mark.parametrize("alg_choice", [0, 1, 2, 3])
+def test_fields_equal_single_steps_and_oracle(mp, tdt, npdt, alg_choice):
+    from oracle import Oracle
+    dims = (37, 29, 13, 1)
+    nf = len(ROLES)
+    base, xs, bxs, bys = _fields(dims, nf, npdt)
+    orc = Oracle(npdt)
+    want = []
+    for f in range(nf):
+        xo = xs[f].copy()
+        orc.step(_variant(mp, alg_choice, ROLES[f]), 2, 1, xo, base["u1"], base["u2"], base["u3"],
+                 base["rhr"], base["h"], bxs[f], bys[f], (0, 0, 0), dims)
+        want.append(xo)
+    with mp.Domain(*dims, dtype=tdt) as d:
+        t = {k: to_dev(v) for k, v in base.items()}
+        x = [to_dev(a) for a in xs]
+        bx = [to_dev(a) for a in bxs]
+        by = [to_dev(a) for a in bys]
+        print(f"ce_advection {tdt} alg_choice {alg_choice}", flush=True)
+        d.ce_advection(x, bx, by, ROLES, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"],
+                       alg_choice=alg_choice)
+        # one field at a time through the single-domain step
+        single = [to_dev(a) for a in xs]
+        for f in range(nf):
+            d.step(single[f], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], bx[f], by[f],
+                   variant=_variant(mp, alg_choice, ROLES[f]))
+        for f in range(nf):
+            got = to_host(x[f])
+            assert bits_equal(interior(got), interior(to_host(single[f]))), f"field {f}"
+            assert bits_equal(interior(got), interior(want[f])), \
+                f"field {f}: " + describe_diff(interior(got), interior(want[f]))
+
+
+@pytest.mark.parametrize("path_name", ["PATH_AUTO", "PATH_STAGES"])
+def test_out_of_place_and_repeated_calls(mp, path_name):
+    dims = (40, 24, 18, 1)
+    nf = 6
+    base, xs, bxs, bys = _fields(dims, nf, np.float64, seed=3)
+    with mp.Domain(*dims) as d:
+        d.set_path(getattr(mp, path_name))
+        t = {k: to_dev(v) for k, v in base.items()}
+        bx = [to_dev(a) for a in bxs]
+        by = [to_dev(a) for a in bys]
+        roles = ROLES[:nf]
+        # three steps in place
+        a = [to_dev(v) for v in xs]
+        for _ in range(3):
+            d.ce_advection(a, bx, by, roles, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"])
+        # three steps alternating two buffer sets out of place (the table is
+        # re-packed when the pointers change)
+        p = [to_dev(v) for v in xs]
+        q = [torch.zeros_like(v) for v in p]
+        keep = [v.clone() for v in p]
+        d.ce_advection(p, bx, by, roles, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], out=q)
+        for f in range(nf):
+            assert torch.equal(p[f], keep[f]), "out-of-place step modified its input"
+        d.ce_advection(q, bx, by, roles, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], out=p)
+        d.ce_advection(p, bx, by, roles, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], out=q)
+        # one field at a time
+        s = [to_dev(v) for v in xs]
+        for f in range(nf):
+            for _ in range(3):
+                d.step(s[f], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], bx[f], by[f],
+                       variant=_variant(mp, 0, roles[f]))
+        for f in range(nf):
+            ref = interior(to_host(s[f]))
+            assert bits_equal(interior(to_host(a[f])), ref), f"in place, field {f}"
+            assert bits_equal(interior(to_host(q[f])), ref), f"out of place, field {f}"
+
+
+def test_liner_is_upwind_only(mp):
+    dims = (21, 17, 11, 1)
+    nf = 4
+    base, xs, bxs, bys = _fields(dims, nf, np.float64, seed=5)
+    with mp.Domain(*dims) as d:
+        t = {k: to_dev(v) for k, v in base.items()}
+        x = [to_dev(a) for a in xs]
+        bx = [to_dev(a) for a in bxs]
+        by = [to_dev(a) for a in bys]
+        d.ce_advection(x, bx, by, ROLES[:nf], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"],
+                       liner=1)
+        for f in range(nf):
+            s = to_dev(xs[f])
+            d.step(s, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], bx[f], by[f],
+                   variant=mp.UPWIND)
+            assert bits_equal(interior(to_host(x[f])), interior(to_host(s))), f"field {f}"
+
+
+def test_rejects_bad_arguments(mp):
+    from advectionhpctester_amd._lib import MpdataError
+    dims = (12, 10, 8, 1)
+    base, xs, bxs, bys = _fields(dims, 2, np.float64, seed=9)
+    with mp.Domain(*dims) as d:
+        t = {k: to_dev(v) for k, v in base.items()}
+        x = [to_dev(a) for a in xs]
+        bx = [to_dev(a) for a in bxs]
+        by = [to_dev(a) for a in bys]
+        args = (t["u1"], t["u2"], t["u3"], t["rhr"], t["h"])
+        with pytest.raises(MpdataError):
+            d.ce_advection(x, bx, by, [0, 2], *args)          # role outside 0/1
+        with pytest.raises(MpdataError):
+            d.ce_advection([x[0], x[0]], bx, by, [1, 1], *args)  # the same field twice
+        with pytest.raises(MpdataError):
+            d.ce_advection(x, bx, by, [1, 1], *args, alg_choice=-1)
+        with pytest.raises(MpdataError):
+            d.ce_advection(x, bx, by, [1, 1], *args, out=[x[1], torch.zeros_like(x[0])])
+        with pytest.raises(MpdataError):
+            d.ce_advection([], [], [], [], *args)
