@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true",
                     help="skip the batched-ensemble measurement (64 tracers on uneven grids)")
     ap.add_argument("--ensemble-steps", type=int, default=100)
+    ap.add_argument("--no-multifield", action="store_true",
+                    help="skip the many-fields-on-one-grid (CE_advection) measurement")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost measurement (128x128x64, 50 steps)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -164,6 +166,11 @@ def main():
         torch.cuda.empty_cache()
         ens = ensemble_bench(a.ensemble_steps)
 
+    multi = None
+    if not a.no_multifield and a.dtype == "f64":
+        torch.cuda.empty_cache()
+        multi = multifield_bench(20)
+
     adj = None
     if not a.no_adjoint and a.dtype == "f64":
         torch.cuda.empty_cache()
@@ -197,6 +204,7 @@ def main():
         "cpu_baseline": cpu,
         "unfused_stages": unfused,
         "ensemble": ens,
+        "multifield": multi,
         "adjoint": adj,
         "checked_fused_equals_stages": checked,
         "wall_s_timed": wall,
@@ -262,6 +270,67 @@ def ensemble_bench(steps, nmember=64, seed=4):
             "sequential_ms_per_step": ms_seq / steps,
             "sequential_value": cells * steps / (ms_seq / 1e3),
             "speedup_graph_vs_sequential": ms_seq / ms_graph}
+
+
+def multifield_bench(steps, size=(256, 256, 128), nfields=8):
+    """Eight fields on one 256x256x128 grid with one velocity (the
+    reference's CE_advection with th, u, v, w, tht, pstr, qv, qc): Fourier
+    fields of different seeds, the headline's tilted rotation at Courant 0.8.
+    All fields gauge (mpdata_alg_choice 1), out of place with two buffer
+    sets: one ce_advection call per step against eight single-field steps;
+    then the reference's default mix (alg_choice 0: five gauge, three
+    standard fields), in place."""
+    import torch
+
+    from advectionhpctester_amd import mpdata, testdata
+    dev = torch.device("cuda:0")
+    np_, mp_, lp_ = size
+    case = testdata.tilted_rotation_case(np_, mp_, lp_, cmax=0.8, device=dev)
+    vel = (case["u1"], case["u2"], case["u3"], case["rhr"], case["h"])
+    roles = [0, 1, 1, 1, 1, 1, 0, 0]
+    xa = [testdata.fourier_field(np_, mp_, lp_, seed=40 + f, device=dev)
+          for f in range(nfields)]
+    xb = [torch.empty_like(t) for t in xa]
+    bx = [case["bcx"]] * nfields
+    by = [case["bcy"]] * nfields
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    cells = np_ * mp_ * lp_ * nfields
+
+    def timed(fn):
+        fn(2)
+        torch.cuda.synchronize()
+        ev0.record()
+        fn(steps)
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / steps
+
+    with mpdata.Domain(np_, mp_, lp_) as d:
+        def together(n):
+            for s in range(n):
+                src, dst = (xa, xb) if s % 2 == 0 else (xb, xa)
+                d.ce_advection(src, bx, by, roles, *vel, alg_choice=1, out=dst)
+
+        def one_by_one(n):
+            for s in range(n):
+                for f in range(nfields):
+                    src, dst = (xa[f], xb[f]) if s % 2 == 0 else (xb[f], xa[f])
+                    d.step_oop(src, dst, *vel, bx[f], by[f])
+
+        def mixed(n):
+            for _ in range(n):
+                d.ce_advection(xa, bx, by, roles, *vel, alg_choice=0)
+
+        ms_together = timed(together)
+        ms_single = timed(one_by_one)
+        ms_mixed = timed(mixed)
+    return {"grid": list(size), "fields": nfields, "steps": steps,
+            "gauge_together_ms_per_step": ms_together,
+            "gauge_together_value": cells / (ms_together / 1e3),
+            "gauge_one_by_one_ms_per_step": ms_single,
+            "speedup_together_vs_one_by_one": ms_single / ms_together,
+            "ce_mix_in_place_ms_per_step": ms_mixed,
+            "ce_mix": "alg_choice 0: th, qv, qc standard (stage kernels), u, v, w, tht, pstr gauge"}
 
 
 def adjoint_bench(nsteps=50, ckpt=10, reps=3):
