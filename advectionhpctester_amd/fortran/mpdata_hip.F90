@@ -43,10 +43,12 @@ MODULE mpdata_hip
    PUBLIC :: mpdata_alloc, mpdata_free, mpdata_put, mpdata_get, mpdata_sync, mpdata_check
    PUBLIC :: upwind3d_gpubc, antidiff3d_gauge_gpubc, antidiff3d_standard_gpubc, advec_dwarf
    PUBLIC :: mpdata_step, mpdata_timeloop, velprdA_to_C, compute_courants, norm_sums
-   PUBLIC :: mpdata_version
+   PUBLIC :: mpdata_version, CE_advection
 
-   ! the module-global context (the reference's module-global scratch)
+   ! the module-global context (the reference's module-global scratch) and
+   ! the domain it was created for (np, mp, lp, ih, ibcx, ibcy, ibcz)
    TYPE(c_ptr), SAVE :: ctx = c_null_ptr
+   INTEGER, SAVE :: ctx_dims(7) = 0
 
    INTERFACE
       FUNCTION c_version() BIND(C, name="mpdata_version")
@@ -176,6 +178,15 @@ MODULE mpdata_hip
          TYPE(c_ptr), VALUE :: stream
          INTEGER(c_int) :: c_courants
       END FUNCTION
+      FUNCTION c_ce_advection(c, alg_choice, liner, nfields, role, x, bcx, bcy, u1, u2, u3, &
+                              rhr, h, stream) BIND(C, name="mpdata_ce_advection"//MP_SFX)
+         IMPORT :: c_ptr, c_int
+         TYPE(c_ptr), VALUE :: c, u1, u2, u3, rhr, h, stream
+         INTEGER(c_int), VALUE :: alg_choice, liner, nfields
+         INTEGER(c_int), INTENT(IN) :: role(*)
+         TYPE(c_ptr), INTENT(IN) :: x(*), bcx(*), bcy(*)
+         INTEGER(c_int) :: c_ce_advection
+      END FUNCTION
       FUNCTION c_norm_sums(c, x, x0, out, stream) BIND(C, name="mpdata_norm_sums"//MP_SFX)
          IMPORT :: c_int, c_ptr, c_double
          TYPE(c_ptr), VALUE :: c, x, x0
@@ -232,6 +243,7 @@ CONTAINS
       CALL mpdata_check(c_ctx_create(INT(np, c_int), INT(mp, c_int), INT(lp, c_int), &
                                      INT(ih, c_int), INT(ibcx, c_int), INT(ibcy, c_int), &
                                      INT(ibcz, c_int), INT(euwp, c_int), ctx), 'mpdata_init')
+      ctx_dims = (/ np, mp, lp, ih, ibcx, ibcy, ibcz /)
    END SUBROUTINE mpdata_init
 
    SUBROUTINE mpdata_finalize()
@@ -346,6 +358,69 @@ CONTAINS
                                       INT(ibcz, c_int), x, x_lr, x_bt, uadv, vadv, wadv, rhr, &
                                       rhoadv, c_null_ptr), 'advec_dwarf')
    END SUBROUTINE advec_dwarf
+
+   ! CE_advection: advection/src_drivers/advection_CEtimestep_twostep_GPUBC.f90:27-42.
+   ! One step of th, u, v, w, tht, pstr and the moist fields present, with
+   ! one velocity (uadv, vadv, wadv), rhr and rhoadv; u, v, w, tht and pstr
+   ! take the antidiffusive pass the reference calls mpdatm3d_antidiff, th
+   ! and the moist fields mpdata3d_antidiff (:145-155, :270-304).  lupdatemulti
+   ! and iflip have no effect on one process without poles.
+   SUBROUTINE CE_advection(lupdatemulti, mpdata_alg_choice, liner, ipoles, ibcx, ibcy, ibcz, &
+                           np, mp, lp, ih, do_serialization_in, do_serialization_out, &
+                           rhr, rhoadv, uadv, vadv, wadv, &
+                           th, th_lr, th_bt, u, u_lr, u_bt, v, v_lr, v_bt, w, w_lr, w_bt, &
+                           tht, tht_lr, tht_bt, pstr, pstr_lr, pstr_bt, &
+                           qv, qv_lr, qv_bt, qc, qc_lr, qc_bt, qr, qr_lr, qr_bt, &
+                           qi, qi_lr, qi_bt, qs, qs_lr, qs_bt, qg, qg_lr, qg_bt)
+      LOGICAL, INTENT(IN) :: lupdatemulti
+      INTEGER, INTENT(IN) :: mpdata_alg_choice, liner, ipoles, ibcx, ibcy, ibcz, np, mp, lp, ih
+      LOGICAL, INTENT(IN), OPTIONAL :: do_serialization_in, do_serialization_out
+      TYPE(c_ptr), INTENT(IN) :: rhr, rhoadv, uadv, vadv, wadv
+      TYPE(c_ptr), INTENT(IN) :: th, th_lr, th_bt, u, u_lr, u_bt, v, v_lr, v_bt, &
+                                 w, w_lr, w_bt, tht, tht_lr, tht_bt, pstr, pstr_lr, pstr_bt
+      TYPE(c_ptr), INTENT(IN), OPTIONAL :: qv, qv_lr, qv_bt, qc, qc_lr, qc_bt, qr, qr_lr, qr_bt, &
+                                           qi, qi_lr, qi_bt, qs, qs_lr, qs_bt, qg, qg_lr, qg_bt
+      TYPE(c_ptr) :: x(12), xl(12), xb(12)
+      INTEGER(c_int) :: role(12)
+      INTEGER :: n
+      IF (ipoles /= 0) THEN
+         WRITE (*, '(A)') 'CE_advection: only the Cartesian case (ipoles = 0) exists'
+         ERROR STOP 1
+      END IF
+      IF (ANY(ctx_dims /= (/ np, mp, lp, ih, ibcx, ibcy, ibcz /))) THEN
+         WRITE (*, '(A)') 'CE_advection: domain or boundary flags differ from mpdata_init'
+         ERROR STOP 1
+      END IF
+      n = 0
+      CALL add(th, th_lr, th_bt, 0)
+      CALL add(u, u_lr, u_bt, 1)
+      CALL add(v, v_lr, v_bt, 1)
+      CALL add(w, w_lr, w_bt, 1)
+      CALL add(tht, tht_lr, tht_bt, 1)
+      CALL add(pstr, pstr_lr, pstr_bt, 1)
+      IF (PRESENT(qv)) CALL add(qv, qv_lr, qv_bt, 0)
+      IF (PRESENT(qc)) CALL add(qc, qc_lr, qc_bt, 0)
+      IF (PRESENT(qr)) CALL add(qr, qr_lr, qr_bt, 0)
+      IF (PRESENT(qi)) CALL add(qi, qi_lr, qi_bt, 0)
+      IF (PRESENT(qs)) CALL add(qs, qs_lr, qs_bt, 0)
+      IF (PRESENT(qg)) CALL add(qg, qg_lr, qg_bt, 0)
+      IF (lupdatemulti .OR. PRESENT(do_serialization_in) .OR. PRESENT(do_serialization_out)) THEN
+         ! no effect here (one process, no serialization)
+      ENDIF
+      CALL mpdata_check(c_ce_advection(ctx, INT(mpdata_alg_choice, c_int), INT(liner, c_int), &
+                                       INT(n, c_int), role, x, xl, xb, uadv, vadv, wadv, rhr, &
+                                       rhoadv, c_null_ptr), 'CE_advection')
+   CONTAINS
+      SUBROUTINE add(f, f_lr, f_bt, r)
+         TYPE(c_ptr), INTENT(IN) :: f, f_lr, f_bt
+         INTEGER, INTENT(IN) :: r
+         n = n + 1
+         x(n) = f
+         xl(n) = f_lr
+         xb(n) = f_bt
+         role(n) = INT(r, c_int)
+      END SUBROUTINE add
+   END SUBROUTINE CE_advection
 
    ! ---- generalised step, time loop, predictor, diagnostics ---------------
    SUBROUTINE mpdata_step(variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy)
