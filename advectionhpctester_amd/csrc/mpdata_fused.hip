@@ -858,11 +858,38 @@ long long items_of(const Geo& g, int clen) {
     return ntx * nty * ((g.lp + clen - 1) / clen);
 }
 
+// Wall time of a step, in k-march steps of one workgroup: the XCDs work in
+// parallel, each on its contiguous share of the tiles, in rounds of per_xcd
+// work items, and an item of clen output planes marches clen + kRamp planes
+// (two planes of xant and one of V/cp/cn recomputed below the chunk, two
+// above it, and the item's set-up).
+constexpr int kRamp = 5;
+long long step_cost(long long ntile, int lp, int clen, long long per_xcd) {
+    const long long nchunk = (lp + clen - 1) / clen;
+    const long long tiles_per_xcd = (ntile + NXCD - 1) / NXCD;
+    const long long rounds = (tiles_per_xcd * nchunk + per_xcd - 1) / per_xcd;
+    return rounds * (clen + kRamp);
+}
+
+template <typename T, int RX, int RY, int MINB>
+int per_cu_of() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fused_gauge2<T, RX, RY, MINB>, RX * RY, 0) !=
+        hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+
 // Tiling and k-chunk of a single-domain step.  32x32 tiles recompute the
-// fewest ring cells but give one workgroup per CU; a domain too small to
-// give every CU two of them takes 32x16 tiles (two workgroups per CU) and
-// the longest k-chunk (64, 32 or 16 planes) that gives every workgroup
-// slot two items, or 16.
+// fewest ring cells but give one workgroup per CU: they are used when the
+// domain gives every CU at least two 64-plane work items, with the chunk
+// length (16 planes or more) of least wall time by step_cost (at
+// 512x512x256: 128 planes, three rounds of items per XCD against 5.75, so
+// six, with 64).  A smaller domain takes 32x16 tiles (two workgroups per
+// CU) and the longest chunk of 64 or 32 planes that gives every workgroup
+// slot two items, else 16 (measured: on smaller domains more, shorter
+// items beat fewer, longer ones).
+template <typename T>
 void choose_tiling(const Geo& g, int& cfg, int& clen) {
     int dev = 0, ncu = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -873,9 +900,24 @@ void choose_tiling(const Geo& g, int& cfg, int& clen) {
     if (env_cfg >= 0) {
         cfg = env_cfg;
         clen = env_clen > 0 ? env_clen : 64;
-    } else if (items_of<32, 32>(g, env_clen > 0 ? env_clen : 64) >= 2LL * ncu) {
+    } else if (items_of<32, 32>(g, 64) >= 2LL * ncu) {
         cfg = 0;
-        clen = env_clen > 0 ? env_clen : 64;
+        clen = env_clen;
+        if (env_clen <= 0) {
+            const long long ntile = items_of<32, 32>(g, g.lp);
+            const long long per_xcd = std::max(1LL, (long long)ncu * per_cu_of<T, 32, 32, 4>() / NXCD);
+            long long best = -1;
+            for (int n = 1; n <= g.lp; n++) {
+                const int cl = (g.lp + n - 1) / n;
+                if (cl < 16) break;
+                const long long cost = step_cost(ntile, g.lp, cl, per_xcd);
+                if (best < 0 || cost < best) {
+                    best = cost;
+                    clen = cl;
+                }
+            }
+            if (best < 0) clen = g.lp;
+        }
     } else {
         cfg = 1;
         clen = env_clen > 0 ? env_clen : 16;
@@ -1066,7 +1108,7 @@ hipError_t Fused<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const 
     }
     hipError_t e = hipSuccess;
     int cfg = 0, cl = 64;
-    choose_tiling(g, cfg, cl);
+    choose_tiling<T>(g, cfg, cl);
     switch (cfg) {
         case 1: e = launch2<T, 32, 16, 4>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
         case 2: e = launch2<T, 32, 16, 2>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s); break;
@@ -1144,8 +1186,8 @@ hipError_t FusedBatch<T>::step(const void* table, int n, int nitem, int cfg, lon
 }
 #undef MPDATA_BATCH_CONFIGS
 
-// Batch tiling for n members on one grid: as choose_tiling with n times
-// the work items (configuration numbers of MPDATA_BATCH_CONFIGS).
+// Batch tiling for n members on one grid (configuration numbers of
+// MPDATA_BATCH_CONFIGS): the single-domain rule with n times the work items.
 template <typename T>
 void FusedBatch<T>::choose_shared(const Geo& g, int n, int& cfg, int& clen) {
     int dev = 0, ncu = 0;
