@@ -361,7 +361,9 @@ k_fused_gauge(Geo g, Off o, const T* __restrict__ x, const T* __restrict__ u1,
 //   sx   x on planes K and K+1 with a 3-cell ring (inflow halo values)
 //   sxa  xant, planes K-2 .. K
 //   sv1, sv2   V1 and V2, planes K-2 and K-1
-//   scp, scn   cp and cn, planes K-3 .. K-1
+//   scp, scn   cp and cn, planes K-2 and K-1
+//   sf1, sf2   limited x/y face fluxes of plane K-2
+//   e_*  (1024-thread tilings) h, u1, u3 of each column on plane K-1
 // The z cross terms of V1 and V2 use pair sums formed one plane at a time:
 // rat4(xa(i-1,k-1), xa(i,k-1), xa(i-1,k+1), xa(i,k+1)) =
 //   ((xa(i,k+1)+xa(i-1,k+1)) - (xa(i,k-1)+xa(i-1,k-1)))*0.25,
@@ -386,18 +388,28 @@ struct XSrc {  // where one sx cell comes from: x, bcx, bcy or nothing
     int base, stride;
 };
 
+// Whether a tiling passes the j-1 column inputs of V2 through LDS (the
+// 1024-thread tilings, one workgroup per CU, have the room; the 512-thread
+// ones run two workgroups per CU and load them).
+template <int RX, int RY>
+struct XJ { static constexpr bool value = RX * RY >= 1024; };
+
 // LDS of one workgroup of the fixed-position kernel.
 template <typename T, int RX, int RY>
 struct FusedLds {
+    static constexpr int NE = XJ<RX, RY>::value ? RY * RX : 1;
     T sx[2][RY + 2][RX + 2];
     T sxa[3][RY][RX];
     T sv1[2][RY][RX];
     T sv2[2][RY][RX];
-    T scp[3][RY][RX];
-    T scn[3][RY][RX];
+    T scp[2][RY][RX];
+    T scn[2][RY][RX];
     T sf1[RY][RX];  // limited flux through the left x-face, plane K-2
     T sf2[RY][RX];  // limited flux through the lower y-face, plane K-2
+    // h, u1 (face i), u3 (faces P, P+1) of each source column on plane P
+    T e_h[NE], e_u1[NE], e_u3a[NE], e_u3b[NE];
 };
+static_assert(sizeof(FusedLds<double, 32, 32>) <= 160 * 1024, "LDS of one CU");
 
 // One work item: output tile `tile` (ntx tiles per row) over the output
 // planes [k0, k1].  Every argument is uniform over the workgroup, so all
@@ -529,11 +541,14 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         // (gi-1, gj) is the source column of the lane to the left wherever
         // V1 is formed (gi >= 2 in the domain, so the left position is in
         // the domain too), and its queues hold h, u2 and u3 of plane P:
-        // taken across lanes.  Column (gi, gj-1) is loaded, before stage A
-        // so that the latency overlaps it, by every lane on every plane (the
-        // plane clamped into the domain; the source column is in the
-        // domain, so its j-1 neighbour is in the arrays), which costs
-        // nothing in a wave that loads anyway; used only where V2 is formed.
+        // taken across lanes.  Column (gi, gj-1) is likewise the source
+        // column of the position below wherever V2 is formed (gj >= 2): with
+        // XJ its h, u1 and u3 go through LDS (written in stage A, read in
+        // B); otherwise they are loaded, before stage A so that the latency
+        // overlaps it, by every lane on every plane (the plane clamped into
+        // the domain; the source column is in the domain, so its j-1
+        // neighbour is in the arrays), which costs nothing in a wave that
+        // loads anyway.
         const int P = K - 1;
         const bool BC_on = P >= max(1, k0 - 1) && P <= min(lp, k1 + 1);
         const int Pc = kc(P);
@@ -543,11 +558,22 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         const T nv1_u3a = from_left_lane(u3q0);
         const T nv1_u3b = from_left_lane(u3q1);
         // column (gi, gj-1)
-        const T nv2_h = ldo(pl(h, SXY, Pc) - o.sx, so);
-        const T nv2_u1a = ldo(pl(u1, FXY, Pc) - o.fx, fo);
-        const T nv2_u1b = ldo(pl(u1, FXY, Pc) - o.fx + 1, fo);
-        const T nv2_u3a = ldo(pl(u3, SXY, Pc) - o.sx, so);
-        const T nv2_u3b = ldo(pl(u3, SXY, Pc + 1) - o.sx, so);
+        T nv2_h = T(0), nv2_u1a = T(0), nv2_u1b = T(0), nv2_u3a = T(0), nv2_u3b = T(0);
+        if constexpr (XJ<RX, RY>::value) {
+            if (BC_on) {
+                const int e = pj * RX + pi;
+                L.e_h[e] = hq1;
+                L.e_u1[e] = u1a0;
+                L.e_u3a[e] = u3q0;
+                L.e_u3b[e] = u3q1;
+            }
+        } else {
+            nv2_h = ldo(pl(h, SXY, Pc) - o.sx, so);
+            nv2_u1a = ldo(pl(u1, FXY, Pc) - o.fx, fo);
+            nv2_u1b = ldo(pl(u1, FXY, Pc) - o.fx + 1, fo);
+            nv2_u3a = ldo(pl(u3, SXY, Pc) - o.sx, so);
+            nv2_u3b = ldo(pl(u3, SXY, Pc + 1) - o.sx, so);
+        }
 
         // ---- A: xant on plane K ------------------------------------------
         T xa_new = T(0);
@@ -592,8 +618,22 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         }
         __syncthreads();
 #define XA(pl, a, b) sxa[(pl) % 3][(b) + 2][(a) + 2]
-#define CP(pl, a, b) scp[(pl) % 3][(b) + 2][(a) + 2]
-#define CN(pl, a, b) scn[(pl) % 3][(b) + 2][(a) + 2]
+#define CP(pl, a, b) scp[(pl) & 1][(b) + 2][(a) + 2]
+#define CN(pl, a, b) scn[(pl) & 1][(b) + 2][(a) + 2]
+        if constexpr (XJ<RX, RY>::value) {
+            if (BC_on && do_v2) {  // pj >= 1 where V2 is formed
+                const int e = (pj - 1) * RX + pi;
+                nv2_h = L.e_h[e];
+                nv2_u1a = L.e_u1[e];
+                nv2_u3a = L.e_u3a[e];
+                nv2_u3b = L.e_u3b[e];
+                // u1 on face i+1 of column j-1: the next position's face i,
+                // except on the last column, whose right neighbour is a
+                // mirrored ring position
+                if (gi < np) nv2_u1b = L.e_u1[e + 1];
+                else nv2_u1b = ldo(pl(u1, FXY, Pc) - o.fx + 1, fo);
+            }
+        }
         const int Q = K - 2;
         const bool Dq = Q >= k0 && Q <= k1;
         // ---- B: pair sums, V1/V2 on plane P=K-1, V3 on face K --------------
@@ -705,16 +745,22 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 rp = (mx - xaP) * hc / (pn(a1p) + pp(a1) + pn(a2p) + pp(a2) + pn(a3p) + pp(a3) + ep);
                 rn = (xaP - mn) * hc / (pp(a1p) + pn(a1) + pp(a2p) + pn(a2) + pp(a3p) + pn(a3) + ep);
             }
-            scp[P % 3][pj][pi] = rp;
-            scn[P % 3][pj][pi] = rn;
+            scp[P & 1][pj][pi] = rp;
+            scn[P & 1][pj][pi] = rn;
         }
+        // the lower z-face flux of the first output plane of the range is
+        // the upper face of plane k0-1, formed here on the step before (the
+        // two-plane cp/cn ring no longer holds plane Q-1 once plane Q is
+        // output); later planes carry it from the previous plane
+        if (Q == k0 - 1 && Q >= 1 && do_d)
+            f3c = gflux(v3q2, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
         // ---- D: the new x on plane Q=K-2 (neighbour data of plane Q was
         //      written in the previous step; own cp/cn of Q+1 just now) ----
         if (D_on) {
             const bool gnd = Q == 1, sky = Q == lp;
             // x/y faces from stage B of this step; the lower z-face flux is
-            // the upper one of plane Q-1 (the same expression), except on
-            // the first plane of the range
+            // the upper one of plane Q-1 (the same expression).  On the sky
+            // plane C did not run this step, so the ring still holds Q-1.
             const T F1 = sf1[pj][pi], F1p = sf1[pj][pi + 1];
             const T F2 = sf2[pj][pi], F2p = sf2[pj + 1][pi];
             T F3, F3p;
@@ -723,8 +769,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 F3 = gflux(-v, CP(Q, li, lj), CN(Q + 1, li, lj), CP(Q + 1, li, lj), CN(Q, li, lj));
                 F3p = gflux(v, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
             } else {
-                if (Q > k0) F3 = f3c;
-                else F3 = gflux(v3q1, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
+                F3 = f3c;
                 if (sky) F3p = gflux(-v3q1, CP(Q - 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q - 1, li, lj));
                 else F3p = gflux(v3q2, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
             }
