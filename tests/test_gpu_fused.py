@@ -51,6 +51,32 @@ def test_fused_matches_stages(mp, tdt, npdt, size):
     assert bits_equal(a, b), describe_diff(a, b)
 
 
+@pytest.mark.parametrize("ih", [2, 3])
+@pytest.mark.parametrize("size", [(45, 19, 7), (33, 40, 9)])
+def test_fused_matches_stages_wide_halo(mp, ih, size):
+    """Halo width ih > 1 (the reference's ih is a parameter): the fused step
+    reads and writes through the same offsets as the stage kernels."""
+    from advectionhpctester_amd.testdata import random_case
+    dims = size + (ih,)
+    c = random_case(*dims, seed=sum(size) + ih, dtype=np.float64)
+    rng = np.random.default_rng(ih)
+    x = c["x"]
+    halo = np.ones_like(x, dtype=bool)
+    halo[ih:-ih, ih:-ih, ih:-ih] = False
+    x[halo] = rng.uniform(-5, 5, int(halo.sum()))
+    with mp.Domain(*dims) as d:
+        t = {k: to_dev(v) for k, v in c.items()}
+        xs = t["x"].clone()
+        xo = torch.full_like(t["x"], float("nan"))
+        d.set_path(mp.PATH_STAGES)
+        d.step(xs, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], t["bcx"], t["bcy"])
+        d.set_path(mp.PATH_FUSED)
+        print(f"fused ih={ih} size={size}", flush=True)
+        d.step_oop(t["x"], xo, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], t["bcx"], t["bcy"])
+        a, b = to_host(xs), to_host(xo)
+    assert bits_equal(interior(a, ih), interior(b, ih)), describe_diff(interior(a, ih), interior(b, ih))
+
+
 @pytest.mark.parametrize("tdt,npdt", DTYPES)
 def test_fused_timeloop_vs_oracle(mp, tdt, npdt):
     """Ping-pong time loop (odd and even counts) against the oracle."""
