@@ -41,8 +41,9 @@ def parse():
     ap.add_argument("--size", type=int, nargs=3, default=(512, 512, 256), metavar=("NP", "MP", "LP"))
     ap.add_argument("--dtype", choices=("f64", "f32"), default="f64")
     ap.add_argument("--path", choices=("auto", "fused", "stages"), default="auto")
-    ap.add_argument("--cpu-steps", type=int, default=2,
-                    help="steps of the CPU baseline sample (0 disables it)")
+    ap.add_argument("--cpu-steps", type=int, default=None,
+                    help="steps of the CPU baseline sample (default 5 through the "
+                         "reference build, 2 through the oracle; 0 disables it)")
     ap.add_argument("--no-check", action="store_true", help="skip the fused-vs-stages check")
     ap.add_argument("--no-unfused", action="store_true",
                     help="skip timing the per-stage (unfused) path beside the fused one")
@@ -156,7 +157,7 @@ def main():
                    "hbm_alg_GBs": alg_bytes / (ms_unf / max(a.steps, 1) / 1e3) / 1e9}
 
     cpu = None
-    if a.cpu_steps > 0:
+    if a.cpu_steps is None or a.cpu_steps > 0:
         cpu = cpu_baseline(case, (np_, mp_, lp_, 1), a.dtype, a.cpu_steps)
 
     ens = None
@@ -386,25 +387,43 @@ def adjoint_bench(nsteps=50, ckpt=10, reps=3):
 
 
 def cpu_baseline(case, dims, dtype, steps):
-    """The oracle (plain C, one thread) on `steps` steps of the same workload."""
+    """The reference's own CPU path (its Fortran routines compiled at -O2 into
+    oracle/_ref/libmpdata_ref.so, one thread: advec_dwarf, mode 4, gauge) on
+    `steps` steps of the same workload; the oracle (plain C restatement) when
+    that build is absent or the run is FP32.  steps=None: 5 steps of the
+    reference or 2 of the oracle (about 10-20 s of CPU work at 512x512x256)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle
+    from oracle import Oracle, Reference
 
     npdt = np.float64 if dtype == "f64" else np.float32
-    host = {k: v.cpu().numpy() for k, v in case.items()}
-    orc = Oracle(npdt)
+    host = {k: np.ascontiguousarray(v.cpu().numpy()) for k, v in case.items()}
     x = host["x"].copy()
+    cells = dims[0] * dims[1] * dims[2]
+    size = f"{dims[0]}x{dims[1]}x{dims[2]} {dtype}"
+    if dtype == "f64" and Reference.available():
+        n = steps or 5
+        ref = Reference()
+        ref.setup(dims[0], dims[1], dims[2], dims[3])
+        t0 = time.perf_counter()
+        for _ in range(n):
+            ref.advec_dwarf(4, 1, x, host["bcx"], host["bcy"], host["u1"], host["u2"], host["u3"],
+                            host["rhr"], host["h"])
+        dt = time.perf_counter() - t0
+        return {"value": cells * n / dt, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"{n} step(s) of the same {size} workload through the reference's "
+                          f"advec_dwarf (its Fortran CPU path compiled -O2 into "
+                          f"oracle/_ref/libmpdata_ref.so, one thread), {dt:.1f} s"}
+    n = steps or 2
+    orc = Oracle(npdt)
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(n):
         orc.step(1, 2, 1, x, host["u1"], host["u2"], host["u3"], host["rhr"], host["h"],
                  host["bcx"], host["bcy"], (0, 0, 0), dims)
     dt = time.perf_counter() - t0
-    cells = dims[0] * dims[1] * dims[2]
-    return {"value": cells * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} step(s) of the same {dims[0]}x{dims[1]}x{dims[2]} {dtype} workload "
-                      f"with oracle/liboracle_mpdata.so (plain C restatement, -O2, one thread), "
-                      f"{dt:.1f} s"}
+    return {"value": cells * n / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{n} step(s) of the same {size} workload with oracle/liboracle_mpdata.so "
+                      f"(plain C restatement, -O2, one thread), {dt:.1f} s"}
 
 
 if __name__ == "__main__":
