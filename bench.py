@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--dtype", choices=("f64", "f32"), default="f64")
     ap.add_argument("--path", choices=("auto", "fused", "stages"), default="auto")
     ap.add_argument("--cpu-steps", type=int, default=None,
-                    help="steps of the CPU baseline sample (default 5 through the "
+                    help="steps of the CPU baseline sample (default 10 through the "
                          "reference build, 2 through the oracle; 0 disables it)")
     ap.add_argument("--no-check", action="store_true", help="skip the fused-vs-stages check")
     ap.add_argument("--no-unfused", action="store_true",
@@ -390,7 +390,7 @@ def cpu_baseline(case, dims, dtype, steps):
     """The reference's own CPU path (its Fortran routines compiled at -O2 into
     oracle/_ref/libmpdata_ref.so, one thread: advec_dwarf, mode 4, gauge) on
     `steps` steps of the same workload; the oracle (plain C restatement) when
-    that build is absent or the run is FP32.  steps=None: 5 steps of the
+    that build is absent or the run is FP32.  steps=None: 10 steps of the
     reference or 2 of the oracle (about 10-20 s of CPU work at 512x512x256)."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -402,7 +402,7 @@ def cpu_baseline(case, dims, dtype, steps):
     cells = dims[0] * dims[1] * dims[2]
     size = f"{dims[0]}x{dims[1]}x{dims[2]} {dtype}"
     if dtype == "f64" and Reference.available():
-        n = steps or 5
+        n = steps or 10
         ref = Reference()
         ref.setup(dims[0], dims[1], dims[2], dims[3])
         t0 = time.perf_counter()
