@@ -90,13 +90,15 @@ def test_no_device_reports_hip_error(lib):
 
 
 def test_product_path_does_not_use_oracle():
-    """The package and its kernels never import, load or link the oracle."""
+    """The package and its kernels never import, load or link the oracle or
+    the compiled reference (oracle/_ref)."""
     pkg = os.path.join(ROOT, "advectionhpctester_amd")
     for dirpath, _, files in os.walk(pkg):
         for f in files:
             if f.endswith((".py", ".hip", ".hpp", ".h", ".f90", ".F90")) or f == "Makefile":
                 text = open(os.path.join(dirpath, f), errors="replace").read()
-                assert not re.search(r"(import\s+oracle|from\s+oracle|liboracle|\borc_)", text), f
+                assert not re.search(r"(import\s+oracle|from\s+oracle|liboracle|\borc_|"
+                                     r"libmpdata_ref|\bref_shim\b|_ref/)", text), f
 
 
 def test_ensemble_create_rejects(lib):
