@@ -153,6 +153,45 @@ def random_case(np_, mp, lp, ih=1, seed=0, dtype=np.float64, cmax=0.3, positive=
     return dict(x=x, u1=u1, u2=u2, u3=u3, h=h, rhr=rhr, bcx=bcx, bcy=bcy)
 
 
+def density_case(np_, mp, lp, ih=1, seed=0, dtype=np.float64, cmax=0.5, dx=100.0):
+    """Inputs of the density step with the predictor driver
+    (mpdata3d-rho-gauge-gpubc.f90:23-91): three time levels (n, n+1/2, n-1)
+    of the A-grid velocity (ox, oy, oz; level n-1 holds momentum ox*h) and
+    of u, v, w; positive density x_incomp with x_in = x_incomp/h; h and pexe
+    in [0.8,1.2]; inflow values; and the grid spacing and time step, dt
+    chosen so that the Courant number at level n is cmax.  Output-only
+    arrays (u1..u3, rhoadv, rhr, pexr1, pexr2, level 1) hold random values,
+    so that a comparison also covers the cells a call must leave alone."""
+    rng = np.random.default_rng(seed)
+
+    def u(kind, lo, hi, lev=None):
+        sh = shape(np_, mp, lp, ih, kind)
+        return rng.uniform(lo, hi, sh if lev is None else (lev,) + sh).astype(dtype)
+
+    h = u(0, 0.8, 1.2)
+    levels = {}
+    for name in ("ox", "oy", "oz"):
+        a = u(0, -10.0, 10.0, 3)
+        a[2] = a[0] * h * (1.0 + rng.uniform(-0.05, 0.05, a[0].shape))
+        levels[name] = a.astype(dtype)
+    for name in ("u", "v", "w"):
+        levels[name] = u(0, -10.0, 10.0, 3)
+    I = (slice(ih, ih + lp), slice(ih, ih + mp), slice(ih, ih + np_))
+    sum_abs = (np.abs(levels["ox"][0][I]) + np.abs(levels["oy"][0][I]) +
+               np.abs(levels["oz"][0][I])).max()
+    dxi = 1.0 / dx
+    dt = cmax / (float(sum_abs) * dxi)
+    x_incomp = u(0, 0.5, 2.0)
+    f = dict(levels, h=h, x_incomp=x_incomp, x_in=(x_incomp / h).astype(dtype),
+             pexe=u(0, 0.8, 1.2), u1=u(1, -1.0, 1.0), u2=u(2, -1.0, 1.0), u3=u(3, -1.0, 1.0),
+             rhoadv=u(0, 0.5, 2.0), rhr=u(0, 0.5, 2.0), pexr1=u(0, 0.5, 2.0),
+             pexr2=u(0, 0.5, 2.0),
+             bcx=rng.uniform(0.5, 2.0, (2, lp, mp)).astype(dtype),
+             bcy=rng.uniform(0.5, 2.0, (2, lp, np_)).astype(dtype))
+    f.update(dxi=dxi, dyi=dxi, dzi=dxi, dt=dt)
+    return f
+
+
 # ---------------------------------------------------------------------------
 # Synthetic workloads (torch, on any device)
 # ---------------------------------------------------------------------------

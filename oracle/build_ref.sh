@@ -49,10 +49,13 @@ direct common/bconditions.f90
 pp common/geometry_datafields_modular.f90
 pp common/geometry.f90
 pp common/diagutils.f90
+pp common/diagnostics.f90
 pp advection/src_algorithms/upwind3d-gpubc.f90
 pp advection/src_algorithms/antidiff3d-gauge-gpubc.f90
 pp advection/src_algorithms/antidiff3d-standard-gpubc.f90
 pp advection/src_algorithms/module_velprd.f90
+pp advection/src_algorithms/velprd_driver.f90
+pp advection/src_algorithms/mpdata3d-rho-gauge-gpubc.f90
 pp advection/src_drivers/advec.f90
 pp advection/src_testdata/advection_test_setup.f90
 $FC $FFLAGS -c "$HERE/ref_shim.f90" -o ref_shim.o

@@ -47,23 +47,28 @@ static inline double copysignr_f64(double a, double b) { return copysign(a, b); 
 static inline float fabsr_f32(float a) { return fabsf(a); }
 static inline float copysignr_f32(float a, float b) { return copysignf(a, b); }
 
-/* ep_nonos = 1.e-10_euwp (common/epsilons.f90:4), HUGE() (diagutils.f90:605) */
+/* ep_nonos = 1.e-10_euwp (common/epsilons.f90:4), HUGE() (diagutils.f90:605),
+ * TINY() (velprd_driver.f90:90) */
 #define R double
 #define S _f64
 #define ORC_EP 1.e-10
 #define ORC_HUGE DBL_MAX
+#define ORC_TINY DBL_MIN
 #include "mpdata_oracle_impl.inc"
 #undef R
 #undef S
 #undef ORC_EP
 #undef ORC_HUGE
+#undef ORC_TINY
 
 #define R float
 #define S _f32
 #define ORC_EP 1.e-10f
 #define ORC_HUGE FLT_MAX
+#define ORC_TINY FLT_MIN
 #include "mpdata_oracle_impl.inc"
 #undef R
 #undef S
 #undef ORC_EP
 #undef ORC_HUGE
+#undef ORC_TINY

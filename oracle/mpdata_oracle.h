@@ -74,7 +74,31 @@ void orc_velprd_rotate##S(R *u, R *v, R *w, R *ox, R *oy, R *oz,             \
                       int ih);                                               \
 void orc_compute_courants##S(const R *ox, const R *oy, const R *oz,          \
                       const R *h, int ntimelevel, R dxi, R dyi, R dzi, R dt, \
-                      int np, int mp, int lp, int ih, R out[7]);
+                      int np, int mp, int lp, int ih, R out[7]);             \
+void orc_velprd_traj0_adapt_dt##S(R *ox, R *oy, R *oz, const R *h, R dxi,    \
+                      R dyi, R dzi, R dt, R dt_ratio, int np, int mp, int lp,\
+                      int ih);                                               \
+int  orc_velprd_driver##S(int *nsubsteps, int *nsubsteps_o,                  \
+                      int lsubstepping, R *dtn, R *dtnold, int ltimeadapt,   \
+                      int lvertsplit, int lcrdiag_nphlf, R crmaxlim,         \
+                      R dtmax, R *ox, R *oy, R *oz, R *u, R *v, R *w, R *u1, \
+                      R *u2, R *u3, const R *h, int ibcx, int ibcy, int ibcz,\
+                      int np, int mp, int lp, int ih, R dxi, R dyi, R dzi,   \
+                      R dt);                                                 \
+int  orc_rho_gauge_core##S(int liner, R *u1, R *u2, R *u3, R *x_in,          \
+                      R *x_incomp, const R *h, R *rhoadv, R *rhr,            \
+                      const R *pexe, R *pexr1, R *pexr2, const R *bcx,       \
+                      const R *bcy, int ibcx, int ibcy, int ibcz, int np,    \
+                      int mp, int lp, int ih);                               \
+int  orc_rho_gauge_step##S(int *nsubsteps, int *nsubsteps_o,                 \
+                      int lsubstepping, R *dtn, R *dtnold, int ltimeadapt,   \
+                      int lvertsplit, int lcrdiag_nphlf, R crmaxlim,         \
+                      R dtmax, R *ox, R *oy, R *oz, R *u, R *v, R *w, R *u1, \
+                      R *u2, R *u3, R *x_in, R *x_incomp, const R *h,        \
+                      R *rhoadv, R *rhr, const R *pexe, R *pexr1, R *pexr2,  \
+                      const R *bcx, const R *bcy, int ibcx, int ibcy,        \
+                      int ibcz, int np, int mp, int lp, int ih, R dxi,       \
+                      R dyi, R dzi, R dt, int liner);
 
 ORC_DECL(double, _f64)
 ORC_DECL(float, _f32)
@@ -82,6 +106,11 @@ ORC_DECL(float, _f32)
 
 /* variant codes shared with the HIP library */
 enum { ORC_UPWIND = 0, ORC_GAUGE = 1, ORC_STANDARD = 2 };
+
+/* status codes of the predictor driver and the density step (0 = done,
+ * -1 = allocation failure) */
+enum { ORC_EPOLICY = 1, ORC_ECFL = 2, ORC_ENOCONV = 3, ORC_EPERIODIC = 4 };
+#define ORC_MAXTRIES 1000
 
 #ifdef __cplusplus
 }

@@ -124,6 +124,44 @@ class Oracle:
         return np.array(list(out), dtype=self.dtype)
 
 
+    # -- density step with the predictor driver (mpdata3d-rho-gauge-gpubc.f90) --
+    RHO_FIELDS = ("ox", "oy", "oz", "u", "v", "w", "u1", "u2", "u3", "x_in", "x_incomp", "h",
+                  "rhoadv", "rhr", "pexe", "pexr1", "pexr2", "bcx", "bcy")
+
+    def rho_gauge(self, f, flags, state, crmaxlim, dtmax, liner, dims, part="all"):
+        """One call of the density step on the dict of arrays ``f`` (updated in
+        place).  flags: dict of lsubstepping, ltimeadapt, lvertsplit,
+        lcrdiag_nphlf; state: dict of nsubsteps, nsubsteps_o, dtn, dtnold
+        (returned updated with the status).  part: "all", "driver" (the
+        predictor driver only) or "core" (the step after the driver, with
+        the C-grid velocities already in u1..u3)."""
+        R = self.real
+        np_, mp, lp, ih = dims
+        self._chk(*[f[k] for k in self.RHO_FIELDS])
+        ns, nso = ctypes.c_int(state["nsubsteps"]), ctypes.c_int(state["nsubsteps_o"])
+        dtn, dtnold = R(state["dtn"]), R(state["dtnold"])
+        fl = [int(bool(flags.get(k, 0))) for k in ("lsubstepping", "ltimeadapt", "lvertsplit",
+                                                    "lcrdiag_nphlf")]
+        sp = [R(f[k]) for k in ("dxi", "dyi", "dzi", "dt")]
+        lev = [_ptr(f[k]) for k in ("ox", "oy", "oz", "u", "v", "w", "u1", "u2", "u3")]
+        sc = [_ptr(f[k]) for k in ("x_in", "x_incomp", "h", "rhoadv", "rhr", "pexe", "pexr1",
+                                   "pexr2", "bcx", "bcy")]
+        if part == "core":
+            rc = self._fn("orc_rho_gauge_core")(liner, *lev[6:], *sc, 0, 0, 0, *dims)
+        elif part == "driver":
+            rc = self._fn("orc_velprd_driver")(ctypes.byref(ns), ctypes.byref(nso), fl[0],
+                                               ctypes.byref(dtn), ctypes.byref(dtnold), fl[1],
+                                               fl[2], fl[3], R(crmaxlim), R(dtmax), *lev,
+                                               _ptr(f["h"]), 0, 0, 0, *dims, *sp)
+        else:
+            rc = self._fn("orc_rho_gauge_step")(ctypes.byref(ns), ctypes.byref(nso), fl[0],
+                                                ctypes.byref(dtn), ctypes.byref(dtnold), fl[1],
+                                                fl[2], fl[3], R(crmaxlim), R(dtmax), *lev, *sc,
+                                                0, 0, 0, *dims, *sp, liner)
+        return {"nsubsteps": ns.value, "nsubsteps_o": nso.value, "dtn": dtn.value,
+                "dtnold": dtnold.value, "status": rc}
+
+
 class Reference:
     """The reference's own routines (FP64 build), through oracle/ref_shim.f90.
 
@@ -176,3 +214,21 @@ class Reference:
         self.lib.ref_compute_courants(*map(_ptr, (ox, oy, oz, h)), ntimelevel, D(dxi), D(dyi),
                                       D(dzi), D(dt), out)
         return np.array(list(out))
+
+    def rho_gauge(self, f, flags, state, crmaxlim, dtmax, liner, only_driver=False):
+        """The reference's mpdata3d_rho_gauge_gpubc (or its velprd_driver alone)
+        on the dict of FP64 arrays ``f``; see :meth:`Oracle.rho_gauge`."""
+        D = ctypes.c_double
+        fl = (ctypes.c_int * 5)(int(bool(flags.get("lupdatemulti", 0))),
+                                *[int(bool(flags.get(k, 0))) for k in
+                                  ("lsubstepping", "ltimeadapt", "lvertsplit", "lcrdiag_nphlf")])
+        isub = (ctypes.c_int * 2)(state["nsubsteps"], state["nsubsteps_o"])
+        rdt = (D * 4)(state["dtn"], state["dtnold"], crmaxlim, dtmax)
+        sp = (D * 4)(*[f[k] for k in ("dxi", "dyi", "dzi", "dt")])
+        lin = ctypes.c_int(liner)
+        names = ("ox", "oy", "oz", "u", "v", "w", "u1", "u2", "u3", "x_in", "x_incomp", "h",
+                 "rhoadv", "rhr", "pexe", "pexr1", "pexr2", "bcx", "bcy")
+        self.lib.ref_rho_gauge(int(bool(only_driver)), fl, isub, rdt,
+                               *[_ptr(f[k]) for k in names], sp, ctypes.byref(lin))
+        return {"nsubsteps": isub[0], "nsubsteps_o": isub[1], "dtn": rdt[0], "dtnold": rdt[1],
+                "status": 0}

@@ -217,6 +217,66 @@ CONTAINS
       out = REAL(c, c_double)
    END SUBROUTINE ref_compute_courants
 
+   ! mpdata3d-rho-gauge-gpubc.f90:23-622 (density step with the predictor
+   ! driver, velprd_driver.f90:14-295).  flags = lupdatemulti, lsubstepping,
+   ! ltimeadapt, lvertsplit, lcrdiag_nphlf; isub = nsubsteps, nsubsteps_o
+   ! (in/out); rdt = dtn, dtnold (in/out), crmaxlim, dtmax; sp = dxi, dyi,
+   ! dzi, dt; liner in/out.  ox..w hold three time levels (0:2).  When
+   ! only_driver /= 0 the predictor driver alone runs.
+   SUBROUTINE ref_rho_gauge(only_driver, flags, isub, rdt, pox, poy, poz, pu, pv, pw, &
+                            pu1, pu2, pu3, px, pxinc, ph, prhoadv, prhr, ppexe, ppexr1, &
+                            ppexr2, pbcx, pbcy, sp, liner) BIND(C, name='ref_rho_gauge')
+      USE module_mpdata3d_rho_gauge_gpubc, ONLY: mpdata3d_rho_gauge_gpubc
+      USE velprd_driver_module, ONLY: velprd_driver
+      INTEGER(c_int), VALUE :: only_driver
+      INTEGER(c_int), INTENT(IN) :: flags(5)
+      INTEGER(c_int), INTENT(INOUT) :: isub(2), liner
+      REAL(c_double), INTENT(INOUT) :: rdt(4)
+      REAL(c_double), INTENT(IN) :: sp(4)
+      TYPE(c_ptr), VALUE :: pox, poy, poz, pu, pv, pw, pu1, pu2, pu3, px, pxinc, ph
+      TYPE(c_ptr), VALUE :: prhoadv, prhr, ppexe, ppexr1, ppexr2, pbcx, pbcy
+      REAL(euwp), POINTER :: u1(:, :, :), u2(:, :, :), u3(:, :, :), x(:, :, :), xinc(:, :, :)
+      REAL(euwp), POINTER :: h(:, :, :), rhoadv(:, :, :), rhr(:, :, :), pexe(:, :, :)
+      REAL(euwp), POINTER :: pexr1(:, :, :), pexr2(:, :, :), bcx(:, :, :), bcy(:, :, :)
+      REAL(euwp), POINTER :: ox(:, :, :, :), oy(:, :, :, :), oz(:, :, :, :)
+      REAL(euwp), POINTER :: u(:, :, :, :), v(:, :, :, :), w(:, :, :, :)
+      LOGICAL :: lf(5)
+      INTEGER :: e(4), nsub, nsubo, lin
+      REAL(euwp) :: dtn, dtnold
+      lf = flags /= 0
+      CALL bind_fields(pu1, pu2, pu3, u1, u2, u3)
+      CALL bind_scalar(ph, h)
+      e = [s_np + 2*s_ih, s_mp + 2*s_ih, s_lp + 2*s_ih, 3]
+      CALL c_f_pointer(pox, ox, e); CALL c_f_pointer(poy, oy, e); CALL c_f_pointer(poz, oz, e)
+      CALL c_f_pointer(pu, u, e); CALL c_f_pointer(pv, v, e); CALL c_f_pointer(pw, w, e)
+      nsub = isub(1); nsubo = isub(2); lin = liner
+      dtn = REAL(rdt(1), euwp); dtnold = REAL(rdt(2), euwp)
+      IF (only_driver /= 0) THEN
+         CALL velprd_driver(nsub, nsubo, lf(2), dtn, dtnold, lf(3), lf(4), lf(5), &
+                            REAL(rdt(3), euwp), REAL(rdt(4), euwp), ox, oy, oz, u, v, w, &
+                            u1, u2, u3, h, 0, s_ibcx, s_ibcy, s_ibcz, s_np, s_mp, s_lp, s_ih, &
+                            REAL(sp(1), euwp), REAL(sp(2), euwp), REAL(sp(3), euwp), &
+                            REAL(sp(4), euwp))
+      ELSE
+         CALL bind_scalar(px, x); CALL bind_scalar(pxinc, xinc)
+         CALL bind_scalar(prhoadv, rhoadv); CALL bind_scalar(prhr, rhr)
+         CALL bind_scalar(ppexe, pexe); CALL bind_scalar(ppexr1, pexr1)
+         CALL bind_scalar(ppexr2, pexr2)
+         CALL c_f_pointer(pbcx, bcx, [s_mp, s_lp, 2])
+         CALL c_f_pointer(pbcy, bcy, [s_np, s_lp, 2])
+         CALL mpdata3d_rho_gauge_gpubc(lf(1), nsub, nsubo, lf(2), dtn, dtnold, lf(3), lf(4), &
+                                       lf(5), REAL(rdt(3), euwp), REAL(rdt(4), euwp), &
+                                       ox, oy, oz, u, v, w, u1, u2, u3, x, xinc, h, rhoadv, rhr, &
+                                       pexe, pexr1, pexr2, bcx, bcy, 0, s_ibcx, s_ibcy, s_ibcz, &
+                                       s_np, s_mp, s_lp, s_ih, REAL(sp(1), euwp), &
+                                       REAL(sp(2), euwp), REAL(sp(3), euwp), REAL(sp(4), euwp), &
+                                       lin, .FALSE., .FALSE.)
+      END IF
+      isub(1) = nsub; isub(2) = nsubo; liner = lin
+      rdt(1) = REAL(dtn, c_double); rdt(2) = REAL(dtnold, c_double)
+      FLUSH (6)
+   END SUBROUTINE ref_rho_gauge
+
    ! -- helpers: bind C storage to Fortran pointers with lower bound 1-ih --
    SUBROUTINE bind_scalar(p, a)
       TYPE(c_ptr), VALUE :: p

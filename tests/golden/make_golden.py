@@ -106,8 +106,47 @@ def dwarf(ref, steps_hash=(1, 50, 100)):
     return hashes, buf.getvalue(), x
 
 
+# Density step with the predictor driver (mpdata3d-rho-gauge-gpubc.f90):
+# name -> (flags, Courant at level n, liner, steps).  Inputs come from
+# testdata.density_case(8, 8, 6, seed=RHO_SEED, cmax=...), so the fixture
+# keeps only the driver state and a SHA-256 of every array after the calls.
+RHO_DIMS = (8, 8, 6, 1)
+RHO_SEED = 5
+RHO_CASES = {
+    "plain": ({}, 0.5, 0, 1),
+    "liner": ({}, 0.5, 1, 1),
+    "nphlf": ({"lcrdiag_nphlf": 1}, 0.5, 0, 1),
+    "substep": ({"lsubstepping": 1}, 1.7, 0, 2),
+    "substep_split": ({"lsubstepping": 1, "lvertsplit": 1}, 1.7, 0, 1),
+    "adapt": ({"ltimeadapt": 1}, 1.3, 0, 2),
+}
+RHO_CRMAX = 0.9
+
+
+def rho_arrays(f):
+    from oracle import Oracle
+    return {k: f[k] for k in Oracle.RHO_FIELDS}
+
+
+def rho_cases(ref):
+    import json
+    ref.setup(*RHO_DIMS[:3], RHO_DIMS[3])
+    out = {}
+    for name, (flags, cmax, liner, steps) in RHO_CASES.items():
+        f = testdata.density_case(*RHO_DIMS, seed=RHO_SEED, cmax=cmax)
+        st = dict(nsubsteps=1, nsubsteps_o=1, dtn=f["dt"], dtnold=f["dt"])
+        for _ in range(steps):
+            st = ref.rho_gauge(f, flags, st, RHO_CRMAX, 10 * f["dt"], liner)
+        out[name] = {"state": {k: st[k] for k in ("nsubsteps", "nsubsteps_o", "dtn", "dtnold")},
+                     "sha256": {k: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+                                for k, a in rho_arrays(f).items()}}
+    return json.dumps(out, indent=1, sort_keys=True)
+
+
 def main():
     ref = Reference()
+    with open(os.path.join(HERE, "rho_reference.json"), "w") as fh:
+        fh.write(rho_cases(ref) + "\n")
     np.savez_compressed(os.path.join(HERE, "stage_n12_l10.npz"), **stage_case(ref, 12, 10, 1))
     np.savez_compressed(os.path.join(HERE, "stage_n9_l6.npz"), **stage_case(ref, 9, 6, 2))
     np.savez_compressed(os.path.join(HERE, "velprd_n8_l6.npz"), **velprd_case(ref, 8, 6, 3))
