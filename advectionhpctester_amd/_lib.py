@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("MPDATA_HIP_LIB") or os.path.join(HERE, "libmpdata_hip.so")
 
 MPDATA_OK, MPDATA_EINVAL, MPDATA_EHIP, MPDATA_ENOMEM, MPDATA_EUNSUPPORTED = 0, 1, 2, 3, 4
+MPDATA_ESTOP = 5
 UPWIND, GAUGE, STANDARD = 0, 1, 2
 PATH_AUTO, PATH_STAGES, PATH_FUSED = 0, 1, 2
 
@@ -31,6 +32,7 @@ class MpdataError(RuntimeError):
 def _proto(real):
     """Argument lists per entry point (without the _f64/_f32 suffix)."""
     R = real
+    IP, RP = ctypes.POINTER(_I), ctypes.POINTER(real)
     return {
         "mpdata_upwind3d_gpubc": [_P] * 12 + [_I] * 8 + [R, _P],
         "mpdata_antidiff3d_gauge_gpubc": [_P, _I] + [_P] * 7 + [_I] * 9 + [_P],
@@ -46,6 +48,12 @@ def _proto(real):
         "mpdata_norm_sums": [_P] * 5,
         "mpdata_ce_advection": [_P, _I, _I, _I] + [_P] * 4 + [_P] * 5 + [_P],
         "mpdata_ce_advection_oop": [_P, _I, _I, _I] + [_P] * 5 + [_P] * 5 + [_P],
+        "mpdata_velprd_traj0_adapt_dt": [_P] * 5 + [R] * 5 + [_P],
+        "mpdata_velprd_driver": [_P, IP, IP, _I, RP, RP, _I, _I, _I, R, R] + [_P] * 10 +
+                                [_I] * 8 + [R] * 4 + [_P],
+        "mpdata_rho_gauge_core": [_P, _I] + [_P] * 13 + [_P],
+        "mpdata3d_rho_gauge_gpubc": [_P, _I, IP, IP, _I, RP, RP, _I, _I, _I, R, R] + [_P] * 19 +
+                                    [_I] * 8 + [R] * 4 + [_I, _P],
     }
 
 

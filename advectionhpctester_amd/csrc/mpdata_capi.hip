@@ -69,6 +69,32 @@ int ensure(mpdata_ctx* c, void** slot, int kind, int elem_bytes) {
 }
 
 
+int check_ctx(const mpdata_ctx* c, int real_bytes) {
+    if (!c) return fail(MPDATA_EINVAL, "null context");
+    if (c->real_bytes != real_bytes)
+        return fail(MPDATA_EINVAL, "context holds %d-byte reals, call needs %d", c->real_bytes,
+                    real_bytes);
+    int dev = -1;
+    if (hipGetDevice(&dev) == hipSuccess && dev != c->device)
+        return fail(MPDATA_EINVAL, "context was created on device %d, current device is %d",
+                    c->device, dev);
+    return MPDATA_OK;
+}
+
+int check_dims(const mpdata_ctx* c, int ipoles, int ibcx, int ibcy, int ibcz, int np, int mp,
+               int lp, int ih) {
+    if (ipoles != 0)
+        return fail(MPDATA_EUNSUPPORTED, "ipoles0=%d: only the Cartesian case (0) exists", ipoles);
+    const Geo& g = c->g;
+    if (np != g.np || mp != g.mp || lp != g.lp || ih != g.ih)
+        return fail(MPDATA_EINVAL, "shape (%d,%d,%d,ih=%d) differs from the context (%d,%d,%d,ih=%d)",
+                    np, mp, lp, ih, g.np, g.mp, g.lp, g.ih);
+    if (ibcx != g.px || ibcy != g.py || ibcz != g.pz)
+        return fail(MPDATA_EINVAL, "boundary flags (%d,%d,%d) differ from the context (%d,%d,%d)",
+                    ibcx, ibcy, ibcz, g.px, g.py, g.pz);
+    return MPDATA_OK;
+}
+
 }  // namespace mpdata
 
 
@@ -96,40 +122,6 @@ int ensure_red(mpdata_ctx* c, long long bytes) {
     c->red_bytes = bytes;
     c->held += bytes;
     return MPDATA_OK;
-}
-
-int check_ctx(const mpdata_ctx* c, int real_bytes) {
-    if (!c) return fail(MPDATA_EINVAL, "null context");
-    if (c->real_bytes != real_bytes)
-        return fail(MPDATA_EINVAL, "context holds %d-byte reals, call needs %d", c->real_bytes,
-                    real_bytes);
-    int dev = -1;
-    if (hipGetDevice(&dev) == hipSuccess && dev != c->device)
-        return fail(MPDATA_EINVAL, "context was created on device %d, current device is %d",
-                    c->device, dev);
-    return MPDATA_OK;
-}
-
-int check_dims(const mpdata_ctx* c, int ipoles, int ibcx, int ibcy, int ibcz, int np, int mp,
-               int lp, int ih) {
-    if (ipoles != 0)
-        return fail(MPDATA_EUNSUPPORTED, "ipoles0=%d: only the Cartesian case (0) exists", ipoles);
-    const Geo& g = c->g;
-    if (np != g.np || mp != g.mp || lp != g.lp || ih != g.ih)
-        return fail(MPDATA_EINVAL, "shape (%d,%d,%d,ih=%d) differs from the context (%d,%d,%d,ih=%d)",
-                    np, mp, lp, ih, g.np, g.mp, g.lp, g.ih);
-    if (ibcx != g.px || ibcy != g.py || ibcz != g.pz)
-        return fail(MPDATA_EINVAL, "boundary flags (%d,%d,%d) differ from the context (%d,%d,%d)",
-                    ibcx, ibcy, ibcz, g.px, g.py, g.pz);
-    return MPDATA_OK;
-}
-
-template <typename... P>
-bool any_null(P... p) {
-    const void* a[] = {(const void*)p...};
-    for (const void* q : a)
-        if (!q) return true;
-    return false;
 }
 
 // One corrective pass on the per-stage kernels, in the reference's stage
@@ -341,6 +333,10 @@ int timeloop_impl(mpdata_ctx* c, int variant, int iters, int nonos, int nt, T* x
     return MPDATA_OK;
 }
 
+}  // namespace
+
+namespace mpdata {
+
 template <typename T>
 int reduce_courants(mpdata_ctx* c, const T* ox, const T* oy, const T* oz, const T* h, int level,
                     T dxi, T dyi, T dzi, T dt, double* out, hipStream_t s) {
@@ -363,6 +359,17 @@ int reduce_courants(mpdata_ctx* c, const T* ox, const T* oy, const T* oz, const 
     }
     return MPDATA_OK;
 }
+
+template int reduce_courants<double>(mpdata_ctx*, const double*, const double*,
+                                      const double*, const double*, int, double, double,
+                                      double, double, double*, hipStream_t);
+template int reduce_courants<float>(mpdata_ctx*, const float*, const float*, const float*,
+                                     const float*, int, float, float, float, float, double*,
+                                     hipStream_t);
+
+}  // namespace mpdata
+
+namespace {
 
 template <typename T>
 int reduce_norms(mpdata_ctx* c, const T* x, const T* x0, double* out, hipStream_t s) {

@@ -64,6 +64,24 @@ struct mpdata_ctx {
 namespace mpdata {
 
 long long field_len(const Geo& g, int kind);
+// Context of the right real size on the current device; shape and boundary
+// flags of a reference-style argument list equal to the context's.
+int check_ctx(const mpdata_ctx* c, int real_bytes);
+int check_dims(const mpdata_ctx* c, int ipoles, int ibcx, int ibcy, int ibcz, int np, int mp,
+               int lp, int ih);
+// Device partials of the Courant maxima reduced on the host (synchronous):
+// out = cr3d, crxy, crxz, cryz, crx, cry, crz (common/diagutils.f90:588-672).
+template <typename T>
+int reduce_courants(mpdata_ctx* c, const T* ox, const T* oy, const T* oz, const T* h, int level,
+                    T dxi, T dyi, T dzi, T dt, double* out, hipStream_t s);
+
+template <typename... P>
+bool any_null(P... p) {
+    const void* a[] = {(const void*)p...};
+    for (const void* q : a)
+        if (!q) return true;
+    return false;
+}
 // Allocates *slot (field of `kind`, elem_bytes per element) unless set.
 int ensure(mpdata_ctx* c, void** slot, int kind, int elem_bytes);
 

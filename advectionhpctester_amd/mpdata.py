@@ -12,6 +12,9 @@ and exposes the reference's subroutines with their argument lists:
                                 advection/src_algorithms/module_velprd.f90:15-364,
                                 velprd_driver.f90:254-270
 * ``compute_courants``          common/diagutils.f90:588-672
+* ``velprd_driver``             advection/src_algorithms/velprd_driver.f90:14-295
+* ``rho_gauge_gpubc`` / ``rho_gauge_core``
+                                advection/src_algorithms/mpdata3d-rho-gauge-gpubc.f90:23-622
 
 plus ``step``/``timeloop`` for the generalised step (iterations, limiter
 switch).  Fields are torch tensors on the GPU with the reference's memory
@@ -281,6 +284,86 @@ class Domain:
             self._ctx, self._levels(u, "u"), self._levels(v, "v"), self._levels(w, "w"),
             self._levels(ox, "ox"), self._levels(oy, "oy"), self._levels(oz, "oz"),
             self._p(h, 0, "h"), self.real(dt_ratio), self._stream()))
+
+    # -- density step with the predictor ----------------------------------------
+    _DRIVER_FLAGS = ("lsubstepping", "ltimeadapt", "lvertsplit", "lcrdiag_nphlf")
+
+    def _driver_args(self, state, flags, crmaxlim, dtmax):
+        unknown = set(flags) - set(self._DRIVER_FLAGS) - {"lupdatemulti"}
+        if unknown:
+            raise MpdataError(_lib.MPDATA_EINVAL, f"unknown flags {sorted(unknown)}")
+        R = self.real
+        box = {"nsubsteps": ctypes.c_int(int(state["nsubsteps"])),
+               "nsubsteps_o": ctypes.c_int(int(state["nsubsteps_o"])),
+               "dtn": R(state["dtn"]), "dtnold": R(state["dtnold"])}
+        fl = [int(bool(flags.get(k, False))) for k in self._DRIVER_FLAGS]
+        head = [ctypes.byref(box["nsubsteps"]), ctypes.byref(box["nsubsteps_o"]), fl[0],
+                ctypes.byref(box["dtn"]), ctypes.byref(box["dtnold"]), fl[1], fl[2], fl[3],
+                R(crmaxlim), R(dtmax)]
+        return box, head
+
+    @staticmethod
+    def _driver_state(state, box):
+        for k, v in box.items():
+            state[k] = v.value
+        return state
+
+    def velprd_driver(self, state, ox, oy, oz, u, v, w, uadv, vadv, wadv, h, dxi, dyi, dzi, dt,
+                      crmaxlim=0.9, dtmax=float("inf"), **flags):
+        """velprd_driver (advection/src_algorithms/velprd_driver.f90:14-295).
+        ``state`` holds nsubsteps, nsubsteps_o, dtn, dtnold and is updated in
+        place (and returned); flags: lsubstepping, ltimeadapt, lvertsplit,
+        lcrdiag_nphlf.  ox..w: (3,)+scalar shape (levels n, n+1/2, n-1).
+        Synchronous (Courant read-backs)."""
+        R = self.real
+        box, head = self._driver_args(state, flags, crmaxlim, dtmax)
+        check(self._fn("mpdata_velprd_driver")(
+            self._ctx, *head, *[self._levels(a, n) for a, n in
+                                ((ox, "ox"), (oy, "oy"), (oz, "oz"), (u, "u"), (v, "v"), (w, "w"))],
+            self._p(uadv, 1, "uadv"), self._p(vadv, 2, "vadv"), self._p(wadv, 3, "wadv"),
+            self._p(h, 0, "h"), 0, *self.ibc, self.np, self.mp, self.lp, self.ih,
+            R(dxi), R(dyi), R(dzi), R(dt), self._stream()))
+        return self._driver_state(state, box)
+
+    def _rho_fields(self, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1, pexr2, bcx, bcy):
+        return [self._p(x_in, 0, "x_in"), self._p(x_incomp, 0, "x_incomp"), self._p(h, 0, "h"),
+                self._p(rhoadv, 0, "rhoadv"), self._p(rhr, 0, "rhr"), self._p(pexe, 0, "pexe"),
+                self._p(pexr1, 0, "pexr1"), self._p(pexr2, 0, "pexr2"), *self._bc(bcx, bcy)]
+
+    def rho_gauge_gpubc(self, state, ox, oy, oz, u, v, w, u1, u2, u3, x_in, x_incomp, h, rhoadv,
+                        rhr, pexe, pexr1, pexr2, bcx, bcy, dxi, dyi, dzi, dt, liner=0,
+                        crmaxlim=0.9, dtmax=float("inf"), **flags):
+        """mpdata3d_rho_gauge_gpubc (advection/src_algorithms/mpdata3d-rho-gauge-
+        gpubc.f90:23-622): the predictor driver, then the density step.  On
+        return u1..u3 hold the total mass fluxes and x_in, x_incomp, rhoadv,
+        rhr, pexr1, pexr2 the new fields; ``state`` as in velprd_driver."""
+        R = self.real
+        box, head = self._driver_args(state, flags, crmaxlim, dtmax)
+        check(self._fn("mpdata3d_rho_gauge_gpubc")(
+            self._ctx, int(bool(flags.get("lupdatemulti", False))), *head,
+            *[self._levels(a, n) for a, n in
+              ((ox, "ox"), (oy, "oy"), (oz, "oz"), (u, "u"), (v, "v"), (w, "w"))],
+            self._p(u1, 1, "u1"), self._p(u2, 2, "u2"), self._p(u3, 3, "u3"),
+            *self._rho_fields(x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1, pexr2, bcx, bcy),
+            0, *self.ibc, self.np, self.mp, self.lp, self.ih, R(dxi), R(dyi), R(dzi), R(dt),
+            int(liner), self._stream()))
+        return self._driver_state(state, box)
+
+    def rho_gauge_core(self, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1, pexr2,
+                       bcx, bcy, liner=0):
+        """The density step after the predictor (u1..u3 = C-grid velocities of
+        n+1/2 on entry, total mass fluxes on return); asynchronous."""
+        check(self._fn("mpdata_rho_gauge_core")(
+            self._ctx, int(liner), self._p(u1, 1, "u1"), self._p(u2, 2, "u2"),
+            self._p(u3, 3, "u3"),
+            *self._rho_fields(x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1, pexr2, bcx, bcy),
+            self._stream()))
+
+    def velprd_traj0_adapt_dt(self, ox, oy, oz, h, dxi, dyi, dzi, dt, dt_ratio):
+        R = self.real
+        check(self._fn("mpdata_velprd_traj0_adapt_dt")(
+            self._ctx, self._levels(ox, "ox"), self._levels(oy, "oy"), self._levels(oz, "oz"),
+            self._p(h, 0, "h"), R(dxi), R(dyi), R(dzi), R(dt), R(dt_ratio), self._stream()))
 
     # -- diagnostics -------------------------------------------------------------
     def compute_courants(self, ox, oy, oz, h, ntimelevel, dxi, dyi, dzi, dt):

@@ -45,6 +45,7 @@ extern "C" {
 #define MPDATA_EHIP 2        /* HIP runtime error                         */
 #define MPDATA_ENOMEM 3      /* device allocation failed                  */
 #define MPDATA_EUNSUPPORTED 4 /* option the reference has no meaning for  */
+#define MPDATA_ESTOP 5       /* a condition on which the reference STOPs  */
 
 /* variant codes (advec.f90:148-211: legacyoptmode 6 -> UPWIND,
  * legacyoptmode 4 with t_adv 1 -> GAUGE, t_adv 2 -> STANDARD) */
@@ -223,6 +224,81 @@ int mpdata_velprd_rotate_f64(mpdata_ctx *ctx, double *u, double *v, double *w, d
                              void *stream);
 int mpdata_velprd_rotate_f32(mpdata_ctx *ctx, float *u, float *v, float *w, float *ox, float *oy,
                              float *oz, const float *h, float dt_ratio, void *stream);
+
+/* velprd_traj0_adapt_dt (module_velprd.f90:91-164): the extrapolation of
+ * velprd_traj0 into level 1 without the C-grid averaging. */
+int mpdata_velprd_traj0_adapt_dt_f64(mpdata_ctx *ctx, double *ox, double *oy, double *oz,
+                                     const double *h, double dxi, double dyi, double dzi,
+                                     double dt, double dt_ratio, void *stream);
+int mpdata_velprd_traj0_adapt_dt_f32(mpdata_ctx *ctx, float *ox, float *oy, float *oz,
+                                     const float *h, float dxi, float dyi, float dzi, float dt,
+                                     float dt_ratio, void *stream);
+
+/* velprd_driver (advection/src_algorithms/velprd_driver.f90:14-295): the
+ * Courant diagnosis, substepping (lsubstepping: nsubsteps and dtn chosen
+ * so that the Courant number stays <= crmaxlim) or adaptive time step
+ * (ltimeadapt: dtn <= dtmax), the predicted C-grid velocities uadv, vadv,
+ * wadv of n+1/2 (halved w for lvertsplit) and the rotation of the time
+ * levels of ox..w.  nsubsteps, nsubsteps_o, dtn, dtnold are host scalars,
+ * read and updated as the reference's INOUT arguments.  The Courant
+ * read-backs synchronise the stream; the last kernels are queued on it.
+ * MPDATA_ESTOP where the reference STOPs (substepping with adaptivity;
+ * Courant/nsubsteps > 10 while substepping) and where its loops would not
+ * end (no admissible step within 1000 trials). */
+int mpdata_velprd_driver_f64(mpdata_ctx *ctx, int *nsubsteps, int *nsubsteps_o,
+                             int lsubstepping, double *dtn, double *dtnold, int ltimeadapt,
+                             int lvertsplit, int lcrdiag_nphlf, double crmaxlim, double dtmax,
+                             double *ox, double *oy, double *oz, double *u, double *v, double *w,
+                             double *uadv, double *vadv, double *wadv, const double *h,
+                             int ipoles0, int ibcx0, int ibcy0, int ibcz0, int np, int mp,
+                             int lp, int ih, double dxi, double dyi, double dzi, double dt,
+                             void *stream);
+int mpdata_velprd_driver_f32(mpdata_ctx *ctx, int *nsubsteps, int *nsubsteps_o,
+                             int lsubstepping, float *dtn, float *dtnold, int ltimeadapt,
+                             int lvertsplit, int lcrdiag_nphlf, float crmaxlim, float dtmax,
+                             float *ox, float *oy, float *oz, float *u, float *v, float *w,
+                             float *uadv, float *vadv, float *wadv, const float *h, int ipoles0,
+                             int ibcx0, int ibcy0, int ibcz0, int np, int mp, int lp, int ih,
+                             float dxi, float dyi, float dzi, float dt, void *stream);
+
+/* ---- density step: advection/src_algorithms/mpdata3d-rho-gauge-gpubc.f90 --
+ * mpdata3d_rho_gauge_gpubc (:23-622) with the reference's argument list
+ * (serialization flags dropped, context first, stream last; liner by
+ * value, as the reference never changes it): the predictor driver above,
+ * then infinite-gauge MPDATA of x_in = x_incomp/h with the predicted u1..u3
+ * and no rhr factor.  On return u1..u3 hold the total mass fluxes (donor
+ * plus limited antidiffusive), x_in the new field, x_incomp = x_in*h,
+ * pexr1 = x_incomp*pexe, pexr2 = x_incomp, rhoadv = x_incomp at level n and
+ * rhr = rhoadv/x_incomp; liner != 0 stops after the upwind pass.  Open
+ * boundaries only (MPDATA_EUNSUPPORTED otherwise).  x_in, x_incomp, rhoadv,
+ * rhr, pexr1, pexr2, h and pexe must be distinct arrays. */
+int mpdata3d_rho_gauge_gpubc_f64(
+    mpdata_ctx *ctx, int lupdatemulti, int *nsubsteps, int *nsubsteps_o, int lsubstepping,
+    double *dtn, double *dtnold, int ltimeadapt, int lvertsplit, int lcrdiag_nphlf,
+    double crmaxlim, double dtmax, double *ox, double *oy, double *oz, double *u, double *v,
+    double *w, double *u1, double *u2, double *u3, double *x_in, double *x_incomp,
+    const double *h, double *rhoadv, double *rhr, const double *pexe, double *pexr1,
+    double *pexr2, const double *bcx, const double *bcy, int ipoles0, int ibcx0, int ibcy0,
+    int ibcz0, int np, int mp, int lp, int ih, double dxi, double dyi, double dzi, double dt,
+    int liner, void *stream);
+int mpdata3d_rho_gauge_gpubc_f32(
+    mpdata_ctx *ctx, int lupdatemulti, int *nsubsteps, int *nsubsteps_o, int lsubstepping,
+    float *dtn, float *dtnold, int ltimeadapt, int lvertsplit, int lcrdiag_nphlf,
+    float crmaxlim, float dtmax, float *ox, float *oy, float *oz, float *u, float *v, float *w,
+    float *u1, float *u2, float *u3, float *x_in, float *x_incomp, const float *h,
+    float *rhoadv, float *rhr, const float *pexe, float *pexr1, float *pexr2, const float *bcx,
+    const float *bcy, int ipoles0, int ibcx0, int ibcy0, int ibcz0, int np, int mp, int lp,
+    int ih, float dxi, float dyi, float dzi, float dt, int liner, void *stream);
+/* The density step after the predictor (mpdata3d-rho-gauge-gpubc.f90:104-
+ * 622), with u1..u3 already holding the C-grid velocities: asynchronous. */
+int mpdata_rho_gauge_core_f64(mpdata_ctx *ctx, int liner, double *u1, double *u2, double *u3,
+                              double *x_in, double *x_incomp, const double *h, double *rhoadv,
+                              double *rhr, const double *pexe, double *pexr1, double *pexr2,
+                              const double *bcx, const double *bcy, void *stream);
+int mpdata_rho_gauge_core_f32(mpdata_ctx *ctx, int liner, float *u1, float *u2, float *u3,
+                              float *x_in, float *x_incomp, const float *h, float *rhoadv,
+                              float *rhr, const float *pexe, float *pexr1, float *pexr2,
+                              const float *bcx, const float *bcy, void *stream);
 
 /* ---- diagnostics (synchronous) --------------------------------------------- */
 /* compute_courants: common/diagutils.f90:588-672.  out[7] (host) = cr3d,
