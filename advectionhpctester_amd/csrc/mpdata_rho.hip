@@ -314,12 +314,17 @@ int driver_impl(mpdata_ctx* c, int* nsub, int* nsubo, int lsub, T* dtn, T* dtnol
     if (lsub && *nsub == 1) d.dt_ratio = (dt / T(1)) / (dt / (T)*nsubo);  // :73
     if (lsub) dtntemp = dt;
     if (ladapt) dtntemp = *dtn;
-    if (lnph) {                                                               // :80-89
+    // :80-89.  With a fixed step the Courant number is only printed by the
+    // reference, so it is not measured (no read-back, the call stays
+    // asynchronous); the extrapolation into level 1 is kept, as its result
+    // is the level the C-grid velocities come from.
+    const bool need_cr = lsub || ladapt;
+    if (lnph) {
         const Geo& g = c->g;
         HIPCHK(Stages<T>::velprd_extrap(g, ox, oy, oz, h, dtntemp * dxi, dtntemp * dyi,
                                         dtntemp * dzi, d.dt_ratio, s));
-        if ((rc = d.courants(2, dtntemp))) return rc;
-    } else {
+        if (need_cr && (rc = d.courants(2, dtntemp))) return rc;
+    } else if (need_cr) {
         if ((rc = d.courants(1, dtntemp))) return rc;
     }
     T crdt = d.cr_for_dt(lvs);

@@ -34,7 +34,8 @@ MODULE mpdata_hip
    INTEGER, PARAMETER, PUBLIC :: euwp = c_double
 #endif
    INTEGER(c_int), PARAMETER, PUBLIC :: MPDATA_OK = 0, MPDATA_EINVAL = 1, MPDATA_EHIP = 2, &
-                                        MPDATA_ENOMEM = 3, MPDATA_EUNSUPPORTED = 4
+                                        MPDATA_ENOMEM = 3, MPDATA_EUNSUPPORTED = 4, &
+                                        MPDATA_ESTOP = 5
    INTEGER(c_int), PARAMETER, PUBLIC :: MPDATA_UPWIND = 0, MPDATA_GAUGE = 1, MPDATA_STANDARD = 2
    INTEGER(c_int), PARAMETER, PUBLIC :: MPDATA_PATH_AUTO = 0, MPDATA_PATH_STAGES = 1, &
                                         MPDATA_PATH_FUSED = 2
@@ -43,7 +44,7 @@ MODULE mpdata_hip
    PUBLIC :: mpdata_alloc, mpdata_free, mpdata_put, mpdata_get, mpdata_sync, mpdata_check
    PUBLIC :: upwind3d_gpubc, antidiff3d_gauge_gpubc, antidiff3d_standard_gpubc, advec_dwarf
    PUBLIC :: mpdata_step, mpdata_timeloop, velprdA_to_C, compute_courants, norm_sums
-   PUBLIC :: mpdata_version, CE_advection
+   PUBLIC :: mpdata_version, CE_advection, mpdata3d_rho_gauge_gpubc, velprd_driver
 
    ! the module-global context (the reference's module-global scratch) and
    ! the domain it was created for (np, mp, lp, ih, ibcx, ibcy, ibcz)
@@ -186,6 +187,39 @@ MODULE mpdata_hip
          INTEGER(c_int), INTENT(IN) :: role(*)
          TYPE(c_ptr), INTENT(IN) :: x(*), bcx(*), bcy(*)
          INTEGER(c_int) :: c_ce_advection
+      END FUNCTION
+      FUNCTION c_velprd_driver(c, nsubsteps, nsubsteps_o, lsubstepping, dtn, dtnold, &
+                               ltimeadapt, lvertsplit, lcrdiag_nphlf, crmaxlim, dtmax, ox, oy, &
+                               oz, u, v, w, uadv, vadv, wadv, h, ipoles0, ibcx0, ibcy0, ibcz0, &
+                               np, mp, lp, ih, dxi, dyi, dzi, dt, stream) &
+            BIND(C, name="mpdata_velprd_driver"//MP_SFX)
+         IMPORT :: c_int, c_ptr, euwp
+         TYPE(c_ptr), VALUE :: c
+         INTEGER(c_int), INTENT(INOUT) :: nsubsteps, nsubsteps_o
+         REAL(euwp), INTENT(INOUT) :: dtn, dtnold
+         INTEGER(c_int), VALUE :: lsubstepping, ltimeadapt, lvertsplit, lcrdiag_nphlf
+         REAL(euwp), VALUE :: crmaxlim, dtmax, dxi, dyi, dzi, dt
+         TYPE(c_ptr), VALUE :: ox, oy, oz, u, v, w, uadv, vadv, wadv, h, stream
+         INTEGER(c_int), VALUE :: ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih
+         INTEGER(c_int) :: c_velprd_driver
+      END FUNCTION
+      FUNCTION c_rho_gauge(c, lupdatemulti, nsubsteps, nsubsteps_o, lsubstepping, dtn, dtnold, &
+                           ltimeadapt, lvertsplit, lcrdiag_nphlf, crmaxlim, dtmax, ox, oy, oz, &
+                           u, v, w, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1, &
+                           pexr2, bcx, bcy, ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih, dxi, &
+                           dyi, dzi, dt, liner, stream) &
+            BIND(C, name="mpdata3d_rho_gauge_gpubc"//MP_SFX)
+         IMPORT :: c_int, c_ptr, euwp
+         TYPE(c_ptr), VALUE :: c
+         INTEGER(c_int), VALUE :: lupdatemulti, lsubstepping, ltimeadapt, lvertsplit, &
+                                  lcrdiag_nphlf, liner
+         INTEGER(c_int), INTENT(INOUT) :: nsubsteps, nsubsteps_o
+         REAL(euwp), INTENT(INOUT) :: dtn, dtnold
+         REAL(euwp), VALUE :: crmaxlim, dtmax, dxi, dyi, dzi, dt
+         TYPE(c_ptr), VALUE :: ox, oy, oz, u, v, w, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr
+         TYPE(c_ptr), VALUE :: pexe, pexr1, pexr2, bcx, bcy, stream
+         INTEGER(c_int), VALUE :: ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih
+         INTEGER(c_int) :: c_rho_gauge
       END FUNCTION
       FUNCTION c_norm_sums(c, x, x0, out, stream) BIND(C, name="mpdata_norm_sums"//MP_SFX)
          IMPORT :: c_int, c_ptr, c_double
@@ -421,6 +455,68 @@ CONTAINS
          role(n) = INT(r, c_int)
       END SUBROUTINE add
    END SUBROUTINE CE_advection
+
+   ! mpdata3d_rho_gauge_gpubc: advection/src_algorithms/mpdata3d-rho-gauge-gpubc.f90:23-34.
+   ! The density step with the predictor driver; ox..w carry three time
+   ! levels (0:2) each.  lupdatemulti has no effect on one process.
+   SUBROUTINE mpdata3d_rho_gauge_gpubc(lupdatemulti, nsubsteps, nsubsteps_o, lsubstepping, &
+                                       dtn, dtnold, ltimeadapt, lvertsplit, lcrdiag_nphlf, &
+                                       crmaxlim, dtmax, ox, oy, oz, u, v, w, u1, u2, u3, &
+                                       x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1, pexr2, &
+                                       bcx, bcy, ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih, &
+                                       dxi, dyi, dzi, dt, liner, do_serialization_in, &
+                                       do_serialization_out)
+      LOGICAL, INTENT(IN) :: lupdatemulti, lsubstepping, ltimeadapt, lvertsplit, lcrdiag_nphlf
+      INTEGER, INTENT(INOUT) :: nsubsteps, nsubsteps_o, liner
+      REAL(euwp), INTENT(INOUT) :: dtn, dtnold
+      REAL(euwp), INTENT(IN) :: crmaxlim, dtmax, dxi, dyi, dzi, dt
+      TYPE(c_ptr), INTENT(IN) :: ox, oy, oz, u, v, w, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr
+      TYPE(c_ptr), INTENT(IN) :: pexe, pexr1, pexr2, bcx, bcy
+      INTEGER, INTENT(IN) :: ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih
+      LOGICAL, INTENT(IN), OPTIONAL :: do_serialization_in, do_serialization_out
+      INTEGER(c_int) :: ns, nso
+      ns = INT(nsubsteps, c_int)
+      nso = INT(nsubsteps_o, c_int)
+      CALL mpdata_check(c_rho_gauge(ctx, flag(lupdatemulti), ns, nso, flag(lsubstepping), dtn, &
+                                    dtnold, flag(ltimeadapt), flag(lvertsplit), &
+                                    flag(lcrdiag_nphlf), crmaxlim, dtmax, ox, oy, oz, u, v, w, &
+                                    u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1, &
+                                    pexr2, bcx, bcy, INT(ipoles0, c_int), INT(ibcx0, c_int), &
+                                    INT(ibcy0, c_int), INT(ibcz0, c_int), INT(np, c_int), &
+                                    INT(mp, c_int), INT(lp, c_int), INT(ih, c_int), dxi, dyi, &
+                                    dzi, dt, INT(liner, c_int), c_null_ptr), &
+                        'mpdata3d_rho_gauge_gpubc')
+      nsubsteps = ns
+      nsubsteps_o = nso
+      IF (PRESENT(do_serialization_in) .OR. PRESENT(do_serialization_out)) THEN
+         ! no serialization here
+      ENDIF
+   END SUBROUTINE mpdata3d_rho_gauge_gpubc
+
+   ! velprd_driver: advection/src_algorithms/velprd_driver.f90:14-19
+   SUBROUTINE velprd_driver(nsubsteps, nsubsteps_o, lsubstepping, dtn, dtnold, ltimeadapt, &
+                            lvertsplit, lcrdiag_nphlf, crmaxlim, dtmax, ox, oy, oz, u, v, w, &
+                            uadv, vadv, wadv, h, ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih, &
+                            dxi, dyi, dzi, dt)
+      INTEGER, INTENT(INOUT) :: nsubsteps, nsubsteps_o
+      LOGICAL, INTENT(IN) :: lsubstepping, ltimeadapt, lvertsplit, lcrdiag_nphlf
+      REAL(euwp), INTENT(INOUT) :: dtn, dtnold
+      REAL(euwp), INTENT(IN) :: crmaxlim, dtmax, dxi, dyi, dzi, dt
+      TYPE(c_ptr), INTENT(IN) :: ox, oy, oz, u, v, w, uadv, vadv, wadv, h
+      INTEGER, INTENT(IN) :: ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih
+      INTEGER(c_int) :: ns, nso
+      ns = INT(nsubsteps, c_int)
+      nso = INT(nsubsteps_o, c_int)
+      CALL mpdata_check(c_velprd_driver(ctx, ns, nso, flag(lsubstepping), dtn, dtnold, &
+                                        flag(ltimeadapt), flag(lvertsplit), flag(lcrdiag_nphlf), &
+                                        crmaxlim, dtmax, ox, oy, oz, u, v, w, uadv, vadv, wadv, &
+                                        h, INT(ipoles0, c_int), INT(ibcx0, c_int), &
+                                        INT(ibcy0, c_int), INT(ibcz0, c_int), INT(np, c_int), &
+                                        INT(mp, c_int), INT(lp, c_int), INT(ih, c_int), dxi, &
+                                        dyi, dzi, dt, c_null_ptr), 'velprd_driver')
+      nsubsteps = ns
+      nsubsteps_o = nso
+   END SUBROUTINE velprd_driver
 
    ! ---- generalised step, time loop, predictor, diagnostics ---------------
    SUBROUTINE mpdata_step(variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy)

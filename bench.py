@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--ensemble-steps", type=int, default=100)
     ap.add_argument("--no-multifield", action="store_true",
                     help="skip the many-fields-on-one-grid (CE_advection) measurement")
+    ap.add_argument("--no-density", action="store_true",
+                    help="skip the density step with the velocity predictor")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost measurement (128x128x64, 50 steps)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -172,6 +174,11 @@ def main():
         torch.cuda.empty_cache()
         multi = multifield_bench(20)
 
+    dens = None
+    if not a.no_density and a.dtype == "f64":
+        torch.cuda.empty_cache()
+        dens = density_bench(20)
+
     adj = None
     if not a.no_adjoint and a.dtype == "f64":
         torch.cuda.empty_cache()
@@ -206,6 +213,7 @@ def main():
         "unfused_stages": unfused,
         "ensemble": ens,
         "multifield": multi,
+        "density": dens,
         "adjoint": adj,
         "checked_fused_equals_stages": checked,
         "wall_s_timed": wall,
@@ -332,6 +340,77 @@ def multifield_bench(steps, size=(256, 256, 128), nfields=8):
             "speedup_together_vs_one_by_one": ms_single / ms_together,
             "ce_mix_in_place_ms_per_step": ms_mixed,
             "ce_mix": "alg_choice 0: th, qv, qc standard (stage kernels), u, v, w, tht, pstr gauge"}
+
+
+# Arrays one call of the density step with a fixed time step must move per
+# cell (FP64 words): reads ox,oy,oz at levels n and n-1, h, u,v,w at levels n
+# and n-1, x_in, x_incomp, pexe (16); writes ox,oy,oz at n+1/2 and n-1, u,v,w
+# at n+1/2 and n-1, the mass fluxes u1,u2,u3, x_in, x_incomp, rhoadv, rhr,
+# pexr1, pexr2 (21).
+DENSITY_ARRAYS = 37
+
+
+def density_bench(steps, size=(256, 256, 128)):
+    """The density step with the velocity predictor (the reference's
+    mpdata3d_rho_gauge_gpubc) on one 256x256x128 FP64 grid: A-grid
+    solid-body rotation about (1,1,1) at Courant 0.8 (dx = dt = 1, steady:
+    level n-1 momentum = level n times h), density x_incomp a positive
+    Fourier field, h = 1 + 0.05 (Fourier field - 1), pexe = 1, zero inflow.
+    Fixed time step: one call = extrapolation to n+1/2, A->C averaging,
+    level rotation and the density MPDATA step, all queued without a host
+    read-back.  Then with substepping at Courant 1.6 (two substeps; each
+    call reads the Courant maxima back to the host)."""
+    import torch
+
+    from advectionhpctester_amd import mpdata, testdata
+    dev = torch.device("cuda:0")
+    np_, mp_, lp_ = size
+    ua, va, wa = testdata.solid_body_a_grid(np_, mp_, lp_, cmax=0.8, device=dev)
+    h = 1.0 + 0.05 * (testdata.fourier_field(np_, mp_, lp_, seed=61, device=dev) - 1.0)
+    xinc = testdata.fourier_field(np_, mp_, lp_, seed=62, device=dev)
+    lev = {}
+    for name, a in (("ox", ua), ("oy", va), ("oz", wa)):
+        t = torch.stack([a, a, a * h]).contiguous()
+        lev[name] = t
+    for name, a in (("u", ua), ("v", va), ("w", wa)):
+        lev[name] = torch.stack([a, a, a]).contiguous()
+    del ua, va, wa
+    f = {"x_in": xinc / h, "x_incomp": xinc, "h": h, "pexe": torch.ones_like(h),
+         "rhoadv": torch.zeros_like(h), "rhr": torch.zeros_like(h),
+         "pexr1": torch.zeros_like(h), "pexr2": torch.zeros_like(h),
+         "bcx": torch.zeros((2, lp_, mp_), dtype=h.dtype, device=dev),
+         "bcy": torch.zeros((2, lp_, np_), dtype=h.dtype, device=dev)}
+    cells = np_ * mp_ * lp_
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {"grid": list(size), "steps": steps, "alg_arrays_per_cell": DENSITY_ARRAYS}
+    with mpdata.Domain(np_, mp_, lp_) as d:
+        u1, u2, u3 = d.zeros(1), d.zeros(2), d.zeros(3)
+
+        def call(state, **flags):
+            d.rho_gauge_gpubc(state, lev["ox"], lev["oy"], lev["oz"], lev["u"], lev["v"],
+                              lev["w"], u1, u2, u3, f["x_in"], f["x_incomp"], f["h"],
+                              f["rhoadv"], f["rhr"], f["pexe"], f["pexr1"], f["pexr2"], f["bcx"],
+                              f["bcy"], 1.0, 1.0, 1.0, 1.0, crmaxlim=0.9, dtmax=1.0, **flags)
+
+        for mode, flags, scale in (("fixed", {}, 1.0), ("substep", {"lsubstepping": 1}, 2.0)):
+            for name in ("ox", "oy", "oz"):
+                lev[name][0] *= scale
+                lev[name][2] *= scale
+            st = {"nsubsteps": 1, "nsubsteps_o": 1, "dtn": 1.0, "dtnold": 1.0}
+            for _ in range(3):
+                call(st, **flags)
+            torch.cuda.synchronize()
+            ev0.record()
+            for _ in range(steps):
+                call(st, **flags)
+            ev1.record()
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1) / steps
+            gbs = DENSITY_ARRAYS * 8 * cells / (ms / 1e3) / 1e9
+            out[mode] = {"ms_per_call": ms, "value": cells / (ms / 1e3),
+                         "alg_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                         "nsubsteps": st["nsubsteps"]}
+    return out
 
 
 def adjoint_bench(nsteps=50, ckpt=10, reps=3):

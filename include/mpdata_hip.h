@@ -240,8 +240,10 @@ int mpdata_velprd_traj0_adapt_dt_f32(mpdata_ctx *ctx, float *ox, float *oy, floa
  * (ltimeadapt: dtn <= dtmax), the predicted C-grid velocities uadv, vadv,
  * wadv of n+1/2 (halved w for lvertsplit) and the rotation of the time
  * levels of ox..w.  nsubsteps, nsubsteps_o, dtn, dtnold are host scalars,
- * read and updated as the reference's INOUT arguments.  The Courant
- * read-backs synchronise the stream; the last kernels are queued on it.
+ * read and updated as the reference's INOUT arguments.  With substepping
+ * or adaptivity the Courant read-backs synchronise the stream; with a fixed
+ * step (neither) the Courant number, which the reference only prints, is
+ * not measured and the call is asynchronous.
  * MPDATA_ESTOP where the reference STOPs (substepping with adaptivity;
  * Courant/nsubsteps > 10 while substepping) and where its loops would not
  * end (no admissible step within 1000 trials). */
