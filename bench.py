@@ -54,6 +54,9 @@ def parse():
                     help="skip the many-fields-on-one-grid (CE_advection) measurement")
     ap.add_argument("--no-density", action="store_true",
                     help="skip the density step with the velocity predictor")
+    ap.add_argument("--density-only", action="store_true",
+                    help="run only the density-step measurement (--steps calls) and print its "
+                         "JSON object")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost measurement (128x128x64, 50 steps)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -87,6 +90,9 @@ def main():
         sys.exit(2)
     if a.copy_probe:
         copy_probe(a)
+        return
+    if a.density_only:
+        print(json.dumps(density_bench(a.steps)), flush=True)
         return
     import torch
 

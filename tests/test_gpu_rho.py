@@ -42,7 +42,8 @@ def _diff(a, b):
     return f"max|diff|={d.max():.3e} at {idx}, {int((d > 0).sum())} differ"
 
 
-def _run_both(mp, dims, f, flags, liner, steps, tdt, npdt, part="all", crmax=0.9):
+def _run_both(mp, dims, f, flags, liner, steps, tdt, npdt, part="all", crmax=0.9,
+              path="PATH_AUTO"):
     from oracle import Oracle
     orc = Oracle(npdt)
     want = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in f.items()}
@@ -51,10 +52,11 @@ def _run_both(mp, dims, f, flags, liner, steps, tdt, npdt, part="all", crmax=0.9
     sg = dict(st0)
     dtmax = 8 * f["dt"]
     with mp.Domain(*dims, dtype=tdt) as d:
+        d.set_path(getattr(mp, path))
         g = {k: to_dev(v) for k, v in f.items() if isinstance(v, np.ndarray)}
         for s in range(steps):
-            print(f"density step {tdt} {dims} flags={flags} liner={liner} part={part} step={s}",
-                  flush=True)
+            print(f"density step {tdt} {dims} flags={flags} liner={liner} part={part} "
+                  f"path={path} step={s}", flush=True)
             so = orc.rho_gauge(want, flags, so, crmax, dtmax, liner, dims, part=part)
             assert so["status"] == 0
             if part == "all":
@@ -94,12 +96,24 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("path", ["PATH_AUTO", "PATH_STAGES"])
 @pytest.mark.parametrize("tdt,npdt", DTYPES)
 @pytest.mark.parametrize("flags,cmax,liner,steps", CASES)
-def test_rho_gauge_equals_oracle(mp, tdt, npdt, flags, cmax, liner, steps):
+def test_rho_gauge_equals_oracle(mp, tdt, npdt, flags, cmax, liner, steps, path):
+    """PATH_AUTO runs the fused density kernel (liner = 0) and the fused
+    extrapolation + rotation; PATH_STAGES one kernel per stage."""
     dims = (37, 29, 13, 1)
     f = _case(dims, 7, cmax, npdt)
-    _run_both(mp, dims, f, flags, liner, steps, tdt, npdt)
+    _run_both(mp, dims, f, flags, liner, steps, tdt, npdt, path=path)
+
+
+@pytest.mark.parametrize("tdt,npdt", DTYPES)
+def test_rho_fused_k_chunks(mp, tdt, npdt):
+    """Enough planes for several k-chunks of the fused kernel (16 planes
+    each), a partial last chunk, tiles cut by both walls."""
+    dims = (45, 19, 37, 1)
+    f = _case(dims, 19, 0.7, npdt)
+    _run_both(mp, dims, f, {}, 0, 2, tdt, npdt, path="PATH_FUSED")
 
 
 @pytest.mark.parametrize("tdt,npdt", DTYPES)
@@ -111,11 +125,12 @@ def test_driver_and_core_alone(mp, tdt, npdt, part, liner):
     _run_both(mp, dims, f, flags, liner, 2, tdt, npdt, part=part)
 
 
-def test_rho_gauge_medium(mp):
+@pytest.mark.parametrize("path", ["PATH_FUSED", "PATH_STAGES"])
+def test_rho_gauge_medium(mp, path):
     """A grid several work-groups wide in every direction, np != mp."""
     dims = (130, 97, 41, 1)
     f = _case(dims, 13, 1.2, np.float64)
-    st = _run_both(mp, dims, f, {"lsubstepping": 1}, 0, 2, torch.float64, np.float64)
+    st = _run_both(mp, dims, f, {"lsubstepping": 1}, 0, 2, torch.float64, np.float64, path=path)
     assert st["nsubsteps"] == 2
 
 
@@ -145,6 +160,13 @@ def test_rho_gauge_errors(mp):
                             g2["u1"], g2["u2"], g2["u3"], g2["h"], f2["dxi"], f2["dyi"],
                             f2["dzi"], f2["dt"], lsubstepping=1)
         assert e.value.code == MPDATA_ESTOP
+    with mp.Domain(*dims) as d:
+        d.set_path(mp.PATH_FUSED)
+        with pytest.raises(MpdataError) as e:   # the fused kernel has no upwind-only mode
+            d.rho_gauge_core(g["u1"], g["u2"], g["u3"], g["x_in"], g["x_incomp"], g["h"],
+                             g["rhoadv"], g["rhr"], g["pexe"], g["pexr1"], g["pexr2"], g["bcx"],
+                             g["bcy"], liner=1)
+        assert e.value.code == MPDATA_EUNSUPPORTED
     with mp.Domain(*dims, ibc=(1, 0, 0)) as d:
         with pytest.raises(MpdataError) as e:
             d.rho_gauge_core(g["u1"], g["u2"], g["u3"], g["x_in"], g["x_incomp"], g["h"],
