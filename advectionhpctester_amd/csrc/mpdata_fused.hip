@@ -3,6 +3,8 @@
 // of the reference's two-step GPUBC path (advection/src_algorithms/
 // upwind3d-gpubc.f90:101-198 and antidiff3d-gauge-gpubc.f90:71-430) in one
 // kernel that reads x, u1, u2, u3, h, rhr once and writes the new x once.
+// The fixed-position kernel also has a density mode (DensIO) for the step of
+// mpdata3d-rho-gauge-gpubc.f90:107-622.
 //
 // Decomposition: one workgroup per TX x TY column tile of the (i,j) plane,
 // marching k from the ground to the sky (2.5-D).  Per plane K the workgroup
@@ -414,14 +416,19 @@ static_assert(sizeof(FusedLds<double, 32, 32>) <= 160 * 1024, "LDS of one CU");
 // One work item: output tile `tile` (ntx tiles per row) over the output
 // planes [k0, k1].  Every argument is uniform over the workgroup, so all
 // threads pass the same barriers.
-template <typename T, int RX, int RY>
+// DENS: the density step (DensIO) -- the upwind divides by h without the
+// rhr factor, the update has no rhr, and the output plane also writes
+// x_incomp, pexr1, pexr2, rhoadv, rhr and the total mass fluxes
+// (mpdata3d-rho-gauge-gpubc.f90:123-192, 377-593).
+template <typename T, int RX, int RY, bool DENS = false>
 __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, int tile, int k0,
                                            int k1, const T* __restrict__ x, const T* __restrict__ u1,
                                            const T* __restrict__ u2, const T* __restrict__ u3,
                                            const T* __restrict__ rhr, const T* __restrict__ h,
                                            const T* __restrict__ bcx, const T* __restrict__ bcy,
                                            T* __restrict__ xo,
-                                           FusedLds<T, RX, RY>& L) {
+                                           FusedLds<T, RX, RY>& L,
+                                           const DensIO<T>* dio = nullptr) {
     constexpr int NT = RX * RY, TX = RX - 4, TY = RY - 4;
     constexpr int SW = RX + 2, SH = RY + 2, NRING = 2 * SW + 2 * (SH - 2);
     // lane t-1 holds region position (pi-1, pj) for every pi >= 1, in the
@@ -511,7 +518,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     T u3q0 = ldo(pl(u3, SXY, fc(Ks - 1)), so), u3q1 = ldo(pl(u3, SXY, fc(Ks)), so);
     T u3q2 = ldo(pl(u3, SXY, fc(Ks + 1)), so);
     T hq1 = ldo(pl(h, SXY, kc(Ks - 1)), so), hq2 = ldo(pl(h, SXY, kc(Ks)), so);
-    T rq2 = ldo(pl(rhr, SXY, kc(Ks)), so);
+    T rq2 = T(0);
+    if constexpr (!DENS) rq2 = ldo(pl(rhr, SXY, kc(Ks)), so);
     T v3q1 = T(0), v3q2 = T(0);
     T xmx0 = T(0), xmn0 = T(0), xmx1 = T(0), xmn1 = T(0);  // x in-plane max/min, planes K-1, K
     T f3c = T(0);  // limited flux through the upper z-face of the last output plane
@@ -597,7 +605,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 f3p = donor(xc, xq3, u3q2);
                 f3 = donor(xq1, xc, u3q1);
             }
-            xa_new = rq2 * (xc - (f1p - f1 + f2p - f2 + f3p - f3) * hinv);
+            if constexpr (DENS) xa_new = (xc - (f1p - f1 + f2p - f2 + f3p - f3) / hq2);
+            else xa_new = rq2 * (xc - (f1p - f1 + f2p - f2 + f3p - f3) * hinv);
             sxa[K % 3][pj][pi] = xa_new;
             // bounds of x for the limiter on plane K: the 7-point stencil,
             // where out-of-domain neighbours are edge copies of the centre
@@ -711,14 +720,16 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             u2b1 = ldo(pl(u2, YXY, kc(Kn)) + o.yx, so);
             u3q2 = ldo(pl(u3, SXY, fc(Kn + 1)), so);
             hq2 = ldo(pl(h, SXY, kc(Kn)), so);
-            rq2 = ldo(pl(rhr, SXY, kc(Kn)), so);
+            if constexpr (!DENS) rq2 = ldo(pl(rhr, SXY, kc(Kn)), so);
         }
         __syncthreads();
         // h of plane P for C, h and rhr of the output plane Q = K-2 for D:
         // loaded here (cache hits) rather than carried in registers
         const bool D_on = do_d && Dq;
         const T hP = ldo(pl(h, SXY, kc(P)), so);
-        const T hQ = ldo(pl(h, SXY, kc(Q)), so), rQ = ldo(pl(rhr, SXY, kc(Q)), so);
+        const T hQ = ldo(pl(h, SXY, kc(Q)), so);
+        T rQ = T(0);
+        if constexpr (!DENS) rQ = ldo(pl(rhr, SXY, kc(Q)), so);
         // ---- C: cp, cn on plane P ------------------------------------------
         if (BC_on) {
             const bool gnd = P == 1, sky = P == lp;
@@ -775,14 +786,70 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             }
             f3c = F3p;
             T* const xoq = pl(xo, SXY, Q);
-            const T r = rQ;
-            sto(xoq, so, r * (XA(Q, li, lj) / r - (F1p - F1 + F2p - F2 + F3p - F3) / hQ));
+            if constexpr (DENS) {
+                const T xnew = (XA(Q, li, lj) - (F1p - F1 + F2p - F2 + F3p - F3) / hQ);
+                sto(xoq, so, xnew);
+                const T xi = xnew * hQ;
+                T* const xincq = pl(dio->xinc, SXY, Q);
+                const T rho_n = ldo(xincq, so);
+                sto(xincq, so, xi);
+                sto(pl(dio->pexr1, SXY, Q), so, xi * ldo(pl(dio->pexe, SXY, Q), so));
+                sto(pl(dio->pexr2, SXY, Q), so, xi);
+                sto(pl(dio->rhoadv, SXY, Q), so, rho_n);
+                sto(pl(dio->rhr, SXY, Q), so, rho_n / xi);
+            } else {
+                const T r = rQ;
+                sto(xoq, so, r * (XA(Q, li, lj) / r - (F1p - F1 + F2p - F2 + F3p - F3) / hQ));
+            }
             if (gi == 1 || gi == np || gj == 1 || gj == mp) {
                 const T xe = ldo(pl(x, SXY, Q), so);
                 if (gi == 1) sto(xoq - 1, so, xe);
                 if (gi == np) sto(xoq + 1, so, xe);
                 if (gj == 1) sto(xoq - o.sx, so, xe);
                 if (gj == mp) sto(xoq + o.sx, so, xe);
+            }
+            if constexpr (DENS) {
+                // total mass fluxes through the low faces of the cell: the
+                // upwind's donor fluxes of plane Q (x of level n, inflow
+                // values at the walls) plus the limited fluxes; the wall
+                // faces as (donor + inflow part) - outflow part.  These reach
+                // the outputs, so the compare-and-select operators of the
+                // reference are used (signed zeros as the reference's).
+                const T* const xq = pl(x, SXY, Q);
+                const T xc = ldo(xq, so);
+                const T xl = gi == 1 ? bcx[(gj - 1) + mp * (Q - 1)] : ldo(xq - 1, so);
+                const T xb = gj == 1 ? bcy[(gi - 1) + np * (Q - 1)] : ldo(xq - o.sx, so);
+                const T* const u1q = pl(u1, FXY, Q);
+                const T* const u2q = pl(u2, YXY, Q);
+                const T f1ns = mpdata::donor(xl, xc, ldo(u1q, fo));
+                const T f2ns = mpdata::donor(xb, xc, ldo(u2q, so));
+                T f3ns;
+                if (gnd) f3ns = -mpdata::donor(xc, ldo(pl(x, SXY, Q + 1), so), ldo(pl(u3, SXY, Q + 1), so));
+                else f3ns = mpdata::donor(ldo(pl(x, SXY, Q - 1), so), xc, ldo(pl(u3, SXY, Q), so));
+                sto(pl(dio->m1, FXY, Q), fo, f1ns + F1);
+                sto(pl(dio->m2, YXY, Q), so, f2ns + F2);
+                sto(pl(dio->m3, SXY, Q), so, f3ns + F3);
+                if (gi == np) {
+                    const T w = mpdata::donor(xc, bcx[(gj - 1) + mp * ((Q - 1) + lp)], ldo(u1q + 1, fo));
+                    const T a = sv1[Q & 1][pj][pi + 1];
+                    sto(pl(dio->m1, FXY, Q) + 1, fo,
+                        w + mpdata::pp(a) * mpdata::min3(T(1), CP(Q, li + 1, lj), CN(Q, li, lj)) -
+                            mpdata::pn(a) * mpdata::min3(T(1), CP(Q, li, lj), CN(Q, li + 1, lj)));
+                }
+                if (gj == mp) {
+                    const T w = mpdata::donor(xc, bcy[(gi - 1) + np * ((Q - 1) + lp)], ldo(u2q + o.yx, so));
+                    const T a = sv2[Q & 1][pj + 1][pi];
+                    sto(pl(dio->m2, YXY, Q) + o.yx, so,
+                        w + mpdata::pp(a) * mpdata::min3(T(1), CP(Q, li, lj + 1), CN(Q, li, lj)) -
+                            mpdata::pn(a) * mpdata::min3(T(1), CP(Q, li, lj), CN(Q, li, lj + 1)));
+                }
+                if (sky) {
+                    const T w = -f3ns;
+                    const T a = -v3q1;
+                    sto(pl(dio->m3, SXY, Q + 1), so,
+                        w + mpdata::pp(a) * mpdata::min3(T(1), CP(Q - 1, li, lj), CN(Q, li, lj)) -
+                            mpdata::pn(a) * mpdata::min3(T(1), CP(Q, li, lj), CN(Q - 1, li, lj)));
+                }
             }
         }
 #undef XA
@@ -996,6 +1063,53 @@ hipError_t launch2(const Geo& g, const Off& o, int clen, const T* x, T* xout, co
     return hipGetLastError();
 }
 
+// The density mode of the fixed-position kernel (DensIO), with the work-item
+// order of k_fused_gauge2.
+template <typename T, int RX, int RY, int MINB>
+__global__ void __launch_bounds__(RX * RY, MINB)
+k_fused_dens(Geo g, Off o, int ntx, int nty, int nchunk, int clen, const T* __restrict__ x,
+             const T* __restrict__ u1, const T* __restrict__ u2, const T* __restrict__ u3,
+             const T* __restrict__ h, const T* __restrict__ bcx, const T* __restrict__ bcy,
+             T* __restrict__ xo, DensIO<T> dio) {
+    __shared__ FusedLds<T, RX, RY> L;
+    const int ntile = ntx * nty;
+    const int nxcd = gridDim.x >= NXCD ? NXCD : 1;
+    const int per_xcd = gridDim.x / nxcd;
+    const int xcd = blockIdx.x % nxcd, slot = blockIdx.x / nxcd;
+    const int nwg = per_xcd * nxcd;
+    if (blockIdx.x >= nwg) return;
+    const int tiles_per_xcd = (ntile + nxcd - 1) / nxcd;
+    for (int r = slot; ; r += per_xcd) {
+        const int chunk = r / tiles_per_xcd, tix = r - chunk * tiles_per_xcd;
+        if (chunk >= nchunk) break;
+        const int tile = xcd * tiles_per_xcd + tix;
+        if (tile >= ntile) continue;
+        const int k0 = 1 + chunk * clen, k1 = min(g.lp, k0 + clen - 1);
+        fused_item<T, RX, RY, true>(g, o, ntx, tile, k0, k1, x, u1, u2, u3, nullptr, h, bcx, bcy,
+                                    xo, L, &dio);
+    }
+}
+
+template <typename T, int RX, int RY, int MINB>
+hipError_t launch_dens(const Geo& g, const Off& o, int clen, const T* x, T* xout, const T* u1,
+                       const T* u2, const T* u3, const T* h, const T* bcx, const T* bcy,
+                       const DensIO<T>& d, hipStream_t s) {
+    const int ntx = (g.np + RX - 5) / (RX - 4), nty = (g.mp + RY - 5) / (RY - 4);
+    const int nchunk = (g.lp + clen - 1) / clen;
+    const int nitem = ntx * nty * nchunk;
+    int dev = 0, ncu = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_dens<T, RX, RY, MINB>,
+                                                         RX * RY, 0);
+    if (e != hipSuccess) return e;
+    const int grid = std::max(1, std::min(nitem, ncu * std::max(per_cu, 1)));
+    k_fused_dens<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>(g, o, ntx, nty, nchunk, clen, x, u1, u2,
+                                                           u3, h, bcx, bcy, xout, d);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Batched step: many domains (ensemble members, tracers) of different shapes
 // in one persistent launch.  The work items of all members form one list
@@ -1137,6 +1251,17 @@ bool Fused<T>::supports(const Geo& g, int variant, int iters, int nonos) {
 }
 
 template <typename T>
+hipError_t FusedDens<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                              const T* u3, const T* h, const T* bcx, const T* bcy,
+                              const DensIO<T>& d, hipStream_t s) {
+    const Off o = make_off(g);
+    int cfg = 0, cl = 64;
+    choose_tiling<T>(g, cfg, cl);
+    if (cfg == 0) return launch_dens<T, 32, 32, 4>(g, o, cl, x, xout, u1, u2, u3, h, bcx, bcy, d, s);
+    return launch_dens<T, 32, 16, 4>(g, o, cl, x, xout, u1, u2, u3, h, bcx, bcy, d, s);
+}
+
+template <typename T>
 const char* Fused<T>::scope() {
     return "the gauge variant with 2 iterations, the limiter on and open boundaries";
 }
@@ -1168,6 +1293,8 @@ hipError_t Fused<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const 
 
 template struct Fused<double>;
 template struct Fused<float>;
+template struct FusedDens<double>;
+template struct FusedDens<float>;
 
 // ---- batched step ------------------------------------------------------------
 // Tile configurations of the batched kernel: (id, RX, RY, waves per SIMD).

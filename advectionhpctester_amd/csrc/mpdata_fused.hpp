@@ -18,6 +18,33 @@ struct Fused {
                            hipStream_t s);
 };
 
+// Extra arrays of the density step (the reference's mpdata3d_rho_gauge_gpubc,
+// advection/src_algorithms/mpdata3d-rho-gauge-gpubc.f90:23-622): x_incomp
+// (read as the density at level n, written as x*h), pexe, the derived
+// outputs, and the total mass fluxes m1..m3 (donor plus limited
+// antidiffusive flux, on faces 1..n+1 of their direction).
+template <typename T>
+struct DensIO {
+    T* xinc;
+    const T* pexe;
+    T *rhoadv, *rhr, *pexr1, *pexr2;
+    T *m1, *m2, *m3;
+};
+
+// The density step after the velocity predictor in the fused kernel: the
+// upwind divides by h and has no rhr factor, the update has none either,
+// and the update plane also writes DensIO's outputs.  x_out / m1..m3 are
+// out of place (x and u are read by neighbouring tiles' stencils); only
+// the cells and faces the reference defines are written (interior and
+// wall-line edge copies of x_out).
+template <typename T>
+struct FusedDens {
+    static bool supports(const Geo& g) { return Fused<T>::supports(g, 1, 2, 1); }
+    static hipError_t step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                           const T* u3, const T* h, const T* bcx, const T* bcy,
+                           const DensIO<T>& d, hipStream_t s);
+};
+
 // One member of a batched step: its grid and fields (x in, xo out).
 template <typename T>
 struct BatchMember {

@@ -12,22 +12,24 @@
 // pexr2, rhoadv and rhr.  Open boundaries only, as the reference's
 // single-process path.
 //
-// Kernels: the fused density step (k_rho_fused, one k-march over column
-// tiles, plus k_rho_copyback into the in-place arrays) and, for liner != 0,
-// the context's per-stage path or arrays too large for 32-bit offsets, the
-// stage kernels: the density upwind (k_rho_upwind: xant, the donor fluxes
-// f*ns including the wall faces, rhoadv), the gauge pseudo-velocity, face
-// and limiter stages shared with the tracer step, and the update
-// (k_rho_update).  The predictor's last extrapolation runs fused with the
-// level rotation (k_extrap_rotate).
+// Kernels: the fused density step is the density mode of the tracer step's
+// fixed-position k-march kernel (FusedDens in mpdata_fused.hip), which
+// writes x and the mass fluxes out of place, followed by k_rho_copyback
+// into the reference's in-place arrays (neighbouring tiles read x and u
+// while tiles write).  For liner != 0, the context's per-stage path or
+// arrays too large for 32-bit offsets the stage kernels run: the density
+// upwind (k_rho_upwind: xant, the donor fluxes f*ns including the wall
+// faces, rhoadv), the gauge pseudo-velocity, face and limiter stages shared
+// with the tracer step, and the update (k_rho_update).  The predictor's
+// last extrapolation runs fused with the level rotation (k_extrap_rotate).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 
 #include "../../include/mpdata_hip.h"
 #include "mpdata_ctx.hpp"
+#include "mpdata_fused.hpp"
 #include "mpdata_stages.hpp"
 #include "mpdata_status.hpp"
 
@@ -184,341 +186,11 @@ __global__ void k_halve(T* __restrict__ a, long long n) {
         a[e] = T(0.5) * a[e];
 }
 
-// ---------------------------------------------------------------------------
-// Fused density step (liner = 0, open boundaries): one workgroup per
-// RTX x RTY column tile marches k from the ground to the sky.  Per plane K
-//   A  xant on plane K over the tile plus a 2-cell ring (upwind, no rhr),
-//   B  V1, V2 on plane P = K-1 (tile + 1-face ring) and V3 on face K,
-//   C  cp, cn on plane P (tile + 1-cell ring),
-//   D  on plane Q = K-2 the limited fluxes, the new x, x_incomp, pexr1,
-//      pexr2, rhoadv, rhr, and the total mass fluxes f*ns + F (the donor
-//      fluxes recomputed from x and u of plane Q),
-// with xant, V3, cp, cn in three-plane LDS rings and V1, V2 in two.  Halo
-// values follow the reference's boundary semantics directly (inflow values
-// for the upwind, mirrored xant, zero wall velocities, edge-copied x for
-// the limiter, zero cp/cn halo), so no halo kernels run; every expression
-// is the stage kernels', operand for operand.  x, read by neighbouring
-// tiles' stencils, and u1..u3, read as velocities by neighbouring tiles,
-// are written to scratch (xn, m1..m3) and copied back by k_rho_copyback;
-// x_incomp is read and written only at its own cell and stays in place.
-constexpr int RTX = 32, RTY = 8;
-
 bool open_all_geo(const Geo& g) { return !g.px && !g.py && !g.pz; }
 
-struct ROff {  // 32-bit element offsets (the host checks every array fits)
-    int sx, sxy, fx, fxy, yx, yxy, ih;
-    __device__ int s(int i, int j, int k) const { return (i + ih - 1) + sx * (j + ih - 1) + sxy * (k + ih - 1); }
-    __device__ int f1(int i, int j, int k) const { return (i + ih - 1) + fx * (j + ih - 1) + fxy * (k + ih - 1); }
-    __device__ int f2(int i, int j, int k) const { return (i + ih - 1) + yx * (j + ih - 1) + yxy * (k + ih - 1); }
-    __device__ int f3(int i, int j, int k) const { return s(i, j, k); }
-};
-
-ROff make_roff(const Geo& g) {
-    ROff o;
-    o.ih = g.ih;
-    o.sx = g.np + 2 * g.ih;
-    o.sxy = o.sx * (g.mp + 2 * g.ih);
-    o.fx = g.np + 1 + 2 * g.ih;
-    o.fxy = o.fx * (g.mp + 2 * g.ih);
-    o.yx = g.np + 2 * g.ih;
-    o.yxy = o.yx * (g.mp + 1 + 2 * g.ih);
-    return o;
-}
-
-bool rho_fusable(const Geo& g) {
-    const long long big = std::max(std::max(len_of<FACEX>(g), len_of<FACEY>(g)), len_of<FACEZ>(g));
-    return open_all_geo(g) && big < (1ll << 31) - 1;
-}
-
-// Output planes per workgroup: short enough that the grid holds several
-// workgroups per CU, long enough that the two xant planes and one V/cp/cn
-// plane recomputed at each chunk start stay a small overhead.
-// MPDATA_RHO_CHUNK overrides it for tuning.
-int rho_chunk(const Geo& g) {
-    static int env = -1;
-    if (env < 0) {
-        const char* e = getenv("MPDATA_RHO_CHUNK");
-        env = e ? std::max(1, atoi(e)) : 0;
-    }
-    if (env > 0) return std::min(env, g.lp);
-    return std::min(16, g.lp);
-}
-
-__device__ __forceinline__ int rmirror(int i, int n) { return i == 0 ? 2 : (i == n + 1 ? n - 1 : i); }
-__device__ __forceinline__ int rclamp(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
-
-template <typename T>
-__global__ void __launch_bounds__(RTX * RTY)
-    k_rho_fused(Geo g, ROff o, const T* __restrict__ x, const T* __restrict__ u1,
-                const T* __restrict__ u2, const T* __restrict__ u3, const T* __restrict__ h,
-                const T* __restrict__ pexe, const T* __restrict__ bcx, const T* __restrict__ bcy,
-                T* __restrict__ xinc, T* __restrict__ rhoadv, T* __restrict__ rhr,
-                T* __restrict__ pexr1, T* __restrict__ pexr2, T* __restrict__ xn,
-                T* __restrict__ m1, T* __restrict__ m2, T* __restrict__ m3, int clen) {
-    constexpr int TX = RTX, TY = RTY, NT = TX * TY;
-    constexpr int AX = TX + 4, AY = TY + 4;    // xant: li in [-2, TX+2)
-    constexpr int V1X = TX + 3, V1Y = TY + 2;  // V1 faces li in [-1, TX+2), rows [-1, TY+1)
-    constexpr int V2X = TX + 2, V2Y = TY + 3;  // V2 cols [-1, TX+1), faces [-1, TY+2)
-    constexpr int CX = TX + 2, CY = TY + 2;    // V3, cp, cn: [-1, TX+1) x [-1, TY+1)
-    __shared__ T s_xa[3][AY][AX];
-    __shared__ T s_v1[2][V1Y][V1X];
-    __shared__ T s_v2[2][V2Y][V2X];
-    __shared__ T s_v3[3][CY][CX];
-    __shared__ T s_cp[3][CY][CX];
-    __shared__ T s_cn[3][CY][CX];
-
-    const int np = g.np, mp = g.mp, lp = g.lp;
-    const int i0 = 1 + blockIdx.x * TX, j0 = 1 + blockIdx.y * TY;
-    // output planes [k0, k1]; their cp, cn and V come from planes k0-1 ..
-    // k1+1 (V3 of faces k0-1 .. k1+2), their xant from k0-2 .. k1+2
-    const int k0 = 1 + blockIdx.z * clen, k1 = min(lp, k0 + clen - 1);
-    const int tid = threadIdx.x;
-    const T ep = Eps<T>::v;
-
-    // x as the upwind reads it: inflow values in the open-wall halo (:113)
-    auto x_up = [&](int gi, int gj, int k) -> T {
-        if (gi == 0) return bcx[(gj - 1) + mp * (k - 1)];
-        if (gi == np + 1) return bcx[(gj - 1) + mp * ((k - 1) + lp)];
-        if (gj == 0) return bcy[(gi - 1) + np * (k - 1)];
-        if (gj == mp + 1) return bcy[(gi - 1) + np * ((k - 1) + lp)];
-        return x[o.s(gi, gj, k)];
-    };
-    // x as the limiter reads it: edge copies (:286)
-    auto x_lim = [&](int gi, int gj, int k) -> T {
-        return x[o.s(rclamp(gi, 1, np), rclamp(gj, 1, mp), k)];
-    };
-
-    for (int K = max(1, k0 - 2); K <= k1 + 2; ++K) {
-        // ---- A: xant on plane K (:123-192) ---------------------------------
-        if (K <= lp) {
-            const bool gnd = K == 1, sky = K == lp;
-            T(*xa)[AX] = s_xa[K % 3];
-            for (int q = tid; q < AX * AY; q += NT) {
-                const int lj = q / AX - 2, li = q % AX - 2;
-                // out-of-domain positions take the mirrored value (:216);
-                // positions beyond the one-cell halo are never used
-                const int gi = rclamp(rmirror(i0 + li, np), 1, np);
-                const int gj = rclamp(rmirror(j0 + lj, mp), 1, mp);
-                const int c = o.s(gi, gj, K);
-                const T xc = x[c];
-                const T f1p = donor(xc, x_up(gi + 1, gj, K), u1[o.f1(gi + 1, gj, K)]);
-                const T f1 = donor(x_up(gi - 1, gj, K), xc, u1[o.f1(gi, gj, K)]);
-                const T f2p = donor(xc, x_up(gi, gj + 1, K), u2[o.f2(gi, gj + 1, K)]);
-                const T f2 = donor(x_up(gi, gj - 1, K), xc, u2[o.f2(gi, gj, K)]);
-                T f3p, f3;
-                if (gnd) {
-                    f3p = donor(xc, x[o.s(gi, gj, K + 1)], u3[o.f3(gi, gj, K + 1)]);
-                    f3 = -f3p;
-                } else if (sky) {
-                    f3 = donor(x[o.s(gi, gj, K - 1)], xc, u3[o.f3(gi, gj, K)]);
-                    f3p = -f3;
-                } else {
-                    f3p = donor(xc, x[o.s(gi, gj, K + 1)], u3[o.f3(gi, gj, K + 1)]);
-                    f3 = donor(x[o.s(gi, gj, K - 1)], xc, u3[o.f3(gi, gj, K)]);
-                }
-                xa[lj + 2][li + 2] = (xc - (f1p - f1 + f2p - f2 + f3p - f3) / h[c]);
-            }
-        }
-        __syncthreads();
-
-        // ---- B: V1, V2 on plane P = K-1, V3 on face K (:225-275) ------------
-        const int P = K - 1;
-        const bool plane_p = P >= max(1, k0 - 1) && P <= lp;
-#define XA(pl, li, lj) s_xa[(pl) % 3][(lj) + 2][(li) + 2]
-        if (plane_p) {
-            const bool zterms = !(P == 1 || P == lp);
-            for (int q = tid; q < V1X * V1Y; q += NT) {
-                const int lj = q / V1X - 1, li = q % V1X - 1;  // face li: cells li-1, li
-                const int gi = i0 + li, gj = j0 + lj;
-                T val = T(0);
-                if (gi >= 2 && gi <= np && gj >= 1 && gj <= mp) {
-                    const T hmx = T(1) / (T(0.5) * (h[o.s(gi - 1, gj, P)] + h[o.s(gi, gj, P)]));
-                    const T s2 = u2[o.f2(gi - 1, gj, P)] + u2[o.f2(gi - 1, gj + 1, P)] +
-                                 u2[o.f2(gi, gj + 1, P)] + u2[o.f2(gi, gj, P)];
-                    T cr = s2 * rat4(XA(P, li - 1, lj - 1), XA(P, li, lj - 1), XA(P, li - 1, lj + 1),
-                                     XA(P, li, lj + 1));
-                    if (zterms) {
-                        const T s3 = u3[o.f3(gi - 1, gj, P)] + u3[o.f3(gi - 1, gj, P + 1)] +
-                                     u3[o.f3(gi, gj, P + 1)] + u3[o.f3(gi, gj, P)];
-                        cr = cr + s3 * rat4(XA(P - 1, li - 1, lj), XA(P - 1, li, lj),
-                                            XA(P + 1, li - 1, lj), XA(P + 1, li, lj));
-                    }
-                    const T a = u1[o.f1(gi, gj, P)];
-                    val = vdyf(XA(P, li - 1, lj), XA(P, li, lj), a, hmx) - T(0.125) * a * hmx * (cr);
-                }
-                s_v1[P % 2][lj + 1][li + 1] = val;
-            }
-            for (int q = tid; q < V2X * V2Y; q += NT) {
-                const int lj = q / V2X - 1, li = q % V2X - 1;  // face lj: cells lj-1, lj
-                const int gi = i0 + li, gj = j0 + lj;
-                T val = T(0);
-                if (gj >= 2 && gj <= mp && gi >= 1 && gi <= np) {
-                    const T hmy = T(1) / (T(0.5) * (h[o.s(gi, gj - 1, P)] + h[o.s(gi, gj, P)]));
-                    const T s1 = u1[o.f1(gi, gj - 1, P)] + u1[o.f1(gi, gj, P)] +
-                                 u1[o.f1(gi + 1, gj, P)] + u1[o.f1(gi + 1, gj - 1, P)];
-                    T cr = s1 * rat4(XA(P, li - 1, lj - 1), XA(P, li - 1, lj), XA(P, li + 1, lj - 1),
-                                     XA(P, li + 1, lj));
-                    if (zterms) {
-                        const T s3 = u3[o.f3(gi, gj - 1, P)] + u3[o.f3(gi, gj, P)] +
-                                     u3[o.f3(gi, gj, P + 1)] + u3[o.f3(gi, gj - 1, P + 1)];
-                        cr = cr + s3 * rat4(XA(P - 1, li, lj - 1), XA(P - 1, li, lj),
-                                            XA(P + 1, li, lj - 1), XA(P + 1, li, lj));
-                    }
-                    const T a = u2[o.f2(gi, gj, P)];
-                    val = vdyf(XA(P, li, lj - 1), XA(P, li, lj), a, hmy) - T(0.125) * a * hmy * (cr);
-                }
-                s_v2[P % 2][lj + 1][li + 1] = val;
-            }
-        }
-        if (K >= max(2, k0 - 1) && K <= lp) {
-            const int F = K;  // face between planes F-1 and F
-            for (int q = tid; q < CX * CY; q += NT) {
-                const int lj = q / CX - 1, li = q % CX - 1;
-                const int gi = i0 + li, gj = j0 + lj;
-                T val = T(0);
-                if (gi >= 1 && gi <= np && gj >= 1 && gj <= mp) {
-                    const T hmz = T(1) / (T(0.5) * (h[o.s(gi, gj, F - 1)] + h[o.s(gi, gj, F)]));
-                    const T s1 = u1[o.f1(gi, gj, F - 1)] + u1[o.f1(gi, gj, F)] +
-                                 u1[o.f1(gi + 1, gj, F)] + u1[o.f1(gi + 1, gj, F - 1)];
-                    const T s2 = u2[o.f2(gi, gj, F - 1)] + u2[o.f2(gi, gj + 1, F - 1)] +
-                                 u2[o.f2(gi, gj + 1, F)] + u2[o.f2(gi, gj, F)];
-                    const T cr = s1 * rat4(XA(F - 1, li - 1, lj), XA(F, li - 1, lj),
-                                           XA(F - 1, li + 1, lj), XA(F, li + 1, lj)) +
-                                 s2 * rat4(XA(F - 1, li, lj - 1), XA(F, li, lj - 1),
-                                           XA(F - 1, li, lj + 1), XA(F, li, lj + 1));
-                    const T a = u3[o.f3(gi, gj, F)];
-                    val = vdyf(XA(F - 1, li, lj), XA(F, li, lj), a, hmz) - T(0.125) * a * hmz * (cr);
-                }
-                s_v3[F % 3][lj + 1][li + 1] = val;
-            }
-        }
-        __syncthreads();
-
-        // ---- C: cp, cn on plane P (:287-370) --------------------------------
-        if (plane_p) {
-            const bool gnd = P == 1, sky = P == lp;
-            for (int q = tid; q < CX * CY; q += NT) {
-                const int lj = q / CX - 1, li = q % CX - 1;
-                const int gi = i0 + li, gj = j0 + lj;
-                T rp = T(0), rn = T(0);
-                if (gi >= 1 && gi <= np && gj >= 1 && gj <= mp) {
-                    T mxo = x_lim(gi - 1, gj, P), mno = mxo;
-                    T mx = XA(P, li - 1, lj), mn = mx;
-#define ACC(v, lo, hi) { T t_ = (v); lo = min2(lo, t_); hi = max2(hi, t_); }
-                    ACC(x[o.s(gi, gj, P)], mno, mxo) ACC(x_lim(gi + 1, gj, P), mno, mxo)
-                    ACC(x_lim(gi, gj - 1, P), mno, mxo) ACC(x_lim(gi, gj + 1, P), mno, mxo)
-                    ACC(XA(P, li, lj), mn, mx) ACC(XA(P, li + 1, lj), mn, mx)
-                    ACC(XA(P, li, lj - 1), mn, mx) ACC(XA(P, li, lj + 1), mn, mx)
-                    if (!gnd) { ACC(x[o.s(gi, gj, P - 1)], mno, mxo) ACC(XA(P - 1, li, lj), mn, mx) }
-                    if (!sky) { ACC(x[o.s(gi, gj, P + 1)], mno, mxo) ACC(XA(P + 1, li, lj), mn, mx) }
-#undef ACC
-                    mx = max2(mx, mxo);
-                    mn = min2(mn, mno);
-                    const T a1p = s_v1[P % 2][lj + 1][li + 2], a1 = s_v1[P % 2][lj + 1][li + 1];
-                    const T a2p = s_v2[P % 2][lj + 2][li + 1], a2 = s_v2[P % 2][lj + 1][li + 1];
-                    T a3p, a3;
-                    if (gnd) {
-                        a3p = s_v3[(P + 1) % 3][lj + 1][li + 1];
-                        a3 = -a3p;
-                    } else if (sky) {
-                        a3 = s_v3[P % 3][lj + 1][li + 1];
-                        a3p = -a3;
-                    } else {
-                        a3p = s_v3[(P + 1) % 3][lj + 1][li + 1];
-                        a3 = s_v3[P % 3][lj + 1][li + 1];
-                    }
-                    const T xc = XA(P, li, lj), hc = h[o.s(gi, gj, P)];
-                    rp = (mx - xc) * hc / (pn(a1p) + pp(a1) + pn(a2p) + pp(a2) + pn(a3p) + pp(a3) + ep);
-                    rn = (xc - mn) * hc / (pp(a1p) + pn(a1) + pp(a2p) + pn(a2) + pp(a3p) + pn(a3) + ep);
-                }
-                s_cp[P % 3][lj + 1][li + 1] = rp;
-                s_cn[P % 3][lj + 1][li + 1] = rn;
-            }
-        }
-        __syncthreads();
-
-        // ---- D: plane Q = K-2 (:377-593) ------------------------------------
-        const int Q = K - 2;
-        if (Q >= k0 && Q <= k1) {
-            const bool gnd = Q == 1, sky = Q == lp;
-            const int li = tid % TX, lj = tid / TX;
-            const int gi = i0 + li, gj = j0 + lj;
-            if (gi <= np && gj <= mp) {
-#define CP(pl, a, b) s_cp[(pl) % 3][(b) + 1][(a) + 1]
-#define CN(pl, a, b) s_cn[(pl) % 3][(b) + 1][(a) + 1]
-                const T v1c = s_v1[Q % 2][lj + 1][li + 1], v1p = s_v1[Q % 2][lj + 1][li + 2];
-                const T v2c = s_v2[Q % 2][lj + 1][li + 1], v2p = s_v2[Q % 2][lj + 2][li + 1];
-                const T F1 = gflux(v1c, CP(Q, li, lj), CN(Q, li - 1, lj), CP(Q, li - 1, lj), CN(Q, li, lj));
-                const T F1p = gflux(v1p, CP(Q, li + 1, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q, li + 1, lj));
-                const T F2 = gflux(v2c, CP(Q, li, lj), CN(Q, li, lj - 1), CP(Q, li, lj - 1), CN(Q, li, lj));
-                const T F2p = gflux(v2p, CP(Q, li, lj + 1), CN(Q, li, lj), CP(Q, li, lj), CN(Q, li, lj + 1));
-                T F3, F3p, v3c;
-                if (gnd) {
-                    v3c = s_v3[(Q + 1) % 3][lj + 1][li + 1];
-                    F3 = gflux(-v3c, CP(Q, li, lj), CN(Q + 1, li, lj), CP(Q + 1, li, lj), CN(Q, li, lj));
-                    F3p = gflux(v3c, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
-                } else if (sky) {
-                    v3c = s_v3[Q % 3][lj + 1][li + 1];
-                    F3 = gflux(v3c, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
-                    F3p = gflux(-v3c, CP(Q - 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q - 1, li, lj));
-                } else {
-                    v3c = s_v3[Q % 3][lj + 1][li + 1];
-                    const T vp = s_v3[(Q + 1) % 3][lj + 1][li + 1];
-                    F3 = gflux(v3c, CP(Q, li, lj), CN(Q - 1, li, lj), CP(Q - 1, li, lj), CN(Q, li, lj));
-                    F3p = gflux(vp, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
-                }
-                const int c = o.s(gi, gj, Q);
-                const T hc = h[c];
-                const T xnew = (XA(Q, li, lj) - (F1p - F1 + F2p - F2 + F3p - F3) / hc);
-                const T xi = xnew * hc;
-                const T rho_n = xinc[c];
-                xn[c] = xnew;
-                xinc[c] = xi;
-                pexr1[c] = xi * pexe[c];
-                pexr2[c] = xi;
-                rhoadv[c] = rho_n;
-                rhr[c] = rho_n / xi;
-                // x halo as the reference leaves it: edge copies of x^n (:286)
-                const T xc = x[c];
-                if (gi == 1) xn[o.s(0, gj, Q)] = xc;
-                if (gi == np) xn[o.s(np + 1, gj, Q)] = xc;
-                if (gj == 1) xn[o.s(gi, 0, Q)] = xc;
-                if (gj == mp) xn[o.s(gi, mp + 1, Q)] = xc;
-                // total mass fluxes: the upwind donor fluxes of plane Q plus
-                // the limited antidiffusive fluxes (:402-410); wall faces as
-                // (f*ns + inflow part) - outflow part (:567-593)
-                const T f1ns = donor(x_up(gi - 1, gj, Q), xc, u1[o.f1(gi, gj, Q)]);
-                const T f2ns = donor(x_up(gi, gj - 1, Q), xc, u2[o.f2(gi, gj, Q)]);
-                T f3ns;
-                if (gnd) f3ns = -donor(xc, x[o.s(gi, gj, Q + 1)], u3[o.f3(gi, gj, Q + 1)]);
-                else f3ns = donor(x[o.s(gi, gj, Q - 1)], xc, u3[o.f3(gi, gj, Q)]);
-                m1[o.f1(gi, gj, Q)] = f1ns + F1;
-                m2[o.f2(gi, gj, Q)] = f2ns + F2;
-                m3[o.f3(gi, gj, Q)] = f3ns + F3;
-                if (gi == np) {
-                    const T w = donor(xc, x_up(np + 1, gj, Q), u1[o.f1(np + 1, gj, Q)]);
-                    m1[o.f1(np + 1, gj, Q)] = w + pp(v1p) * min3(T(1), CP(Q, li + 1, lj), CN(Q, li, lj)) -
-                                              pn(v1p) * min3(T(1), CP(Q, li, lj), CN(Q, li + 1, lj));
-                }
-                if (gj == mp) {
-                    const T w = donor(xc, x_up(gi, mp + 1, Q), u2[o.f2(gi, mp + 1, Q)]);
-                    m2[o.f2(gi, mp + 1, Q)] = w + pp(v2p) * min3(T(1), CP(Q, li, lj + 1), CN(Q, li, lj)) -
-                                              pn(v2p) * min3(T(1), CP(Q, li, lj), CN(Q, li, lj + 1));
-                }
-                if (sky) {
-                    const T w = -f3ns;
-                    const T a = -v3c;
-                    m3[o.f3(gi, gj, Q + 1)] = w + pp(a) * min3(T(1), CP(Q - 1, li, lj), CN(Q, li, lj)) -
-                                              pn(a) * min3(T(1), CP(Q, li, lj), CN(Q - 1, li, lj));
-                }
-#undef CP
-#undef CN
-            }
-        }
-#undef XA
-        __syncthreads();
-    }
-}
+// The fused density step runs in the fixed-position kernel of the tracer
+// step (FusedDens, mpdata_fused.hip) where its 32-bit offsets apply.
+bool rho_fusable(const Geo& g) { return open_all_geo(g) && FusedDens<double>::supports(g); }
 
 // The fused step's results back into the reference's in-place arrays: x_in
 // on the interior and the four wall lines (the cells the step defines), and
@@ -623,12 +295,8 @@ int rho_core(mpdata_ctx* c, int liner, T* u1, T* u2, T* u3, T* x, T* xinc, const
         for (int d = 0; d < 3; d++)
             if ((rc = ensure(c, &S.f[d], FACEX + d, sizeof(T)))) return rc;
         T *xn = (T*)S.xnext, *m1 = (T*)S.f[0], *m2 = (T*)S.f[1], *m3 = (T*)S.f[2];
-        const int clen = rho_chunk(g);
-        dim3 grid((g.np + RTX - 1) / RTX, (g.mp + RTY - 1) / RTY, (g.lp + clen - 1) / clen);
-        k_rho_fused<T><<<grid, RTX * RTY, 0, s>>>(g, make_roff(g), x, u1, u2, u3, h, pexe, bcx, bcy,
-                                                  xinc, rhoadv, rhr, pexr1, pexr2, xn, m1, m2, m3,
-                                                  clen);
-        HIPCHK(hipGetLastError());
+        const DensIO<T> dio{xinc, pexe, rhoadv, rhr, pexr1, pexr2, m1, m2, m3};
+        HIPCHK(FusedDens<T>::step(g, x, xn, u1, u2, u3, h, bcx, bcy, dio, s));
         dim3 cgrid((g.np + 2 + 63) / 64, (g.mp + 2 + 3) / 4, g.lp);
         k_rho_copyback<T><<<cgrid, cell_block(), 0, s>>>(g, xn, m1, m2, m3, x, u1, u2, u3);
         HIPCHK(hipGetLastError());

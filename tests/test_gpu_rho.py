@@ -108,10 +108,13 @@ def test_rho_gauge_equals_oracle(mp, tdt, npdt, flags, cmax, liner, steps, path)
 
 
 @pytest.mark.parametrize("tdt,npdt", DTYPES)
-def test_rho_fused_k_chunks(mp, tdt, npdt):
-    """Enough planes for several k-chunks of the fused kernel (16 planes
-    each), a partial last chunk, tiles cut by both walls."""
-    dims = (45, 19, 37, 1)
+@pytest.mark.parametrize("cfg,chunk", [("0", "8"), ("1", "16"), ("1", "5")])
+def test_rho_fused_tilings(mp, tdt, npdt, cfg, chunk, monkeypatch):
+    """Both tilings of the fused kernel (32x32 and 32x16 lanes), several
+    k-chunks with a partial last one, tiles cut by both walls."""
+    monkeypatch.setenv("MPDATA_FUSED_CFG", cfg)
+    monkeypatch.setenv("MPDATA_FUSED_CHUNK", chunk)
+    dims = (70, 45, 30, 1)
     f = _case(dims, 19, 0.7, npdt)
     _run_both(mp, dims, f, {}, 0, 2, tdt, npdt, path="PATH_FUSED")
 
