@@ -129,6 +129,16 @@ def test_driver_and_core_alone(mp, tdt, npdt, part, liner):
 
 
 @pytest.mark.parametrize("path", ["PATH_FUSED", "PATH_STAGES"])
+@pytest.mark.parametrize("ih", [2, 3])
+def test_rho_gauge_wide_halo(mp, path, ih):
+    """Halo width above one: the step reads and writes the first halo layer
+    only, the outer layers keep their values."""
+    dims = (33, 26, 17, ih)
+    f = _case(dims, 23, 1.5, np.float64)
+    _run_both(mp, dims, f, {"lsubstepping": 1}, 0, 2, torch.float64, np.float64, path=path)
+
+
+@pytest.mark.parametrize("path", ["PATH_FUSED", "PATH_STAGES"])
 def test_rho_gauge_medium(mp, path):
     """A grid several work-groups wide in every direction, np != mp."""
     dims = (130, 97, 41, 1)
