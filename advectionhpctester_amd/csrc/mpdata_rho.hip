@@ -186,11 +186,11 @@ __global__ void k_halve(T* __restrict__ a, long long n) {
         a[e] = T(0.5) * a[e];
 }
 
-bool open_all_geo(const Geo& g) { return !g.px && !g.py && !g.pz; }
+bool open_all(const Geo& g) { return !g.px && !g.py && !g.pz; }
 
 // The fused density step runs in the fixed-position kernel of the tracer
 // step (FusedDens, mpdata_fused.hip) where its 32-bit offsets apply.
-bool rho_fusable(const Geo& g) { return open_all_geo(g) && FusedDens<double>::supports(g); }
+bool rho_fusable(const Geo& g) { return open_all(g) && FusedDens<double>::supports(g); }
 
 // The fused step's results back into the reference's in-place arrays: x_in
 // on the interior and the four wall lines (the cells the step defines), and
@@ -258,8 +258,6 @@ __global__ void __launch_bounds__(256)
 
 dim3 cell_block() { return dim3(64, 4, 1); }
 dim3 cell_grid(const Geo& g) { return dim3((g.np + 63) / 64, (g.mp + 3) / 4, g.lp); }
-
-bool open_all(const Geo& g) { return !g.px && !g.py && !g.pz; }
 
 // The density step after the predictor (:104-622 without the driver call).
 // ox(.,2) = ox(.,0)*h (:104-108) is not repeated here: the driver's level
