@@ -418,6 +418,31 @@ def slotted_cylinders(np_, mp, lp, noise=1e-4, seed=2, ih=1):
 # ---------------------------------------------------------------------------
 # Batched ensemble: many tracers on uneven grids
 # ---------------------------------------------------------------------------
+def density_workload(np_=256, mp=256, lp=128, cmax=0.8, ih=1, device="cpu"):
+    """The density-step benchmark workload (torch, FP64): A-grid solid-body
+    rotation about (1,1,1) at Courant cmax with dx = dt = 1 and steady time
+    levels (level n-1 momentum = level n times h), density x_incomp a
+    positive Fourier field, h = 1 + 0.05 (Fourier field - 1), pexe = 1,
+    zero inflow.  Returns the (3,)+scalar-shaped level arrays ox..w, the
+    scalar fields, bcx, bcy and zero-filled outputs."""
+    torch = _torch()
+    ua, va, wa = solid_body_a_grid(np_, mp, lp, cmax=cmax, ih=ih, device=device)
+    h = 1.0 + 0.05 * (fourier_field(np_, mp, lp, seed=61, ih=ih, device=device) - 1.0)
+    xinc = fourier_field(np_, mp, lp, seed=62, ih=ih, device=device)
+    f = {}
+    for name, a in (("ox", ua), ("oy", va), ("oz", wa)):
+        f[name] = torch.stack([a, a, a * h]).contiguous()
+    for name, a in (("u", ua), ("v", va), ("w", wa)):
+        f[name] = torch.stack([a, a, a]).contiguous()
+    z = lambda kind: torch.zeros(shape(np_, mp, lp, ih, kind), dtype=torch.float64,  # noqa: E731
+                                 device=device)
+    f.update(x_in=xinc / h, x_incomp=xinc, h=h, pexe=torch.ones_like(h), rhoadv=z(0), rhr=z(0),
+             pexr1=z(0), pexr2=z(0), u1=z(1), u2=z(2), u3=z(3),
+             bcx=torch.zeros((2, lp, mp), dtype=torch.float64, device=device),
+             bcy=torch.zeros((2, lp, np_), dtype=torch.float64, device=device))
+    return f
+
+
 def ensemble_shapes(nmember=64, seed=4, lo=32, hi=256, step=16):
     """Per-member (np, mp, lp), each drawn from lo..hi in steps of `step`."""
     rng = np.random.default_rng(seed)

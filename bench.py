@@ -358,50 +358,34 @@ DENSITY_ARRAYS = 37
 
 def density_bench(steps, size=(256, 256, 128)):
     """The density step with the velocity predictor (the reference's
-    mpdata3d_rho_gauge_gpubc) on one 256x256x128 FP64 grid: A-grid
-    solid-body rotation about (1,1,1) at Courant 0.8 (dx = dt = 1, steady:
-    level n-1 momentum = level n times h), density x_incomp a positive
-    Fourier field, h = 1 + 0.05 (Fourier field - 1), pexe = 1, zero inflow.
-    Fixed time step: one call = extrapolation to n+1/2, A->C averaging,
-    level rotation and the density MPDATA step, all queued without a host
-    read-back.  Then with substepping at Courant 1.6 (two substeps; each
-    call reads the Courant maxima back to the host)."""
+    mpdata3d_rho_gauge_gpubc) on one 256x256x128 FP64 grid
+    (testdata.density_workload: A-grid solid-body rotation about (1,1,1) at
+    Courant 0.8, dx = dt = 1, steady levels; density x_incomp a positive
+    Fourier field, h = 1 + 0.05 (Fourier field - 1), pexe = 1, zero
+    inflow).  Fixed time step: one call = extrapolation to n+1/2, A->C
+    averaging, level rotation and the density MPDATA step, all queued
+    without a host read-back.  Then with substepping at Courant 1.6 (two
+    substeps; each call reads the Courant maxima back to the host)."""
     import torch
 
     from advectionhpctester_amd import mpdata, testdata
     dev = torch.device("cuda:0")
     np_, mp_, lp_ = size
-    ua, va, wa = testdata.solid_body_a_grid(np_, mp_, lp_, cmax=0.8, device=dev)
-    h = 1.0 + 0.05 * (testdata.fourier_field(np_, mp_, lp_, seed=61, device=dev) - 1.0)
-    xinc = testdata.fourier_field(np_, mp_, lp_, seed=62, device=dev)
-    lev = {}
-    for name, a in (("ox", ua), ("oy", va), ("oz", wa)):
-        t = torch.stack([a, a, a * h]).contiguous()
-        lev[name] = t
-    for name, a in (("u", ua), ("v", va), ("w", wa)):
-        lev[name] = torch.stack([a, a, a]).contiguous()
-    del ua, va, wa
-    f = {"x_in": xinc / h, "x_incomp": xinc, "h": h, "pexe": torch.ones_like(h),
-         "rhoadv": torch.zeros_like(h), "rhr": torch.zeros_like(h),
-         "pexr1": torch.zeros_like(h), "pexr2": torch.zeros_like(h),
-         "bcx": torch.zeros((2, lp_, mp_), dtype=h.dtype, device=dev),
-         "bcy": torch.zeros((2, lp_, np_), dtype=h.dtype, device=dev)}
+    f = testdata.density_workload(np_, mp_, lp_, device=dev)
     cells = np_ * mp_ * lp_
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     out = {"grid": list(size), "steps": steps, "alg_arrays_per_cell": DENSITY_ARRAYS}
     with mpdata.Domain(np_, mp_, lp_) as d:
-        u1, u2, u3 = d.zeros(1), d.zeros(2), d.zeros(3)
-
         def call(state, **flags):
-            d.rho_gauge_gpubc(state, lev["ox"], lev["oy"], lev["oz"], lev["u"], lev["v"],
-                              lev["w"], u1, u2, u3, f["x_in"], f["x_incomp"], f["h"],
-                              f["rhoadv"], f["rhr"], f["pexe"], f["pexr1"], f["pexr2"], f["bcx"],
-                              f["bcy"], 1.0, 1.0, 1.0, 1.0, crmaxlim=0.9, dtmax=1.0, **flags)
+            d.rho_gauge_gpubc(state, f["ox"], f["oy"], f["oz"], f["u"], f["v"], f["w"], f["u1"],
+                              f["u2"], f["u3"], f["x_in"], f["x_incomp"], f["h"], f["rhoadv"],
+                              f["rhr"], f["pexe"], f["pexr1"], f["pexr2"], f["bcx"], f["bcy"],
+                              1.0, 1.0, 1.0, 1.0, crmaxlim=0.9, dtmax=1.0, **flags)
 
         for mode, flags, scale in (("fixed", {}, 1.0), ("substep", {"lsubstepping": 1}, 2.0)):
             for name in ("ox", "oy", "oz"):
-                lev[name][0] *= scale
-                lev[name][2] *= scale
+                f[name][0] *= scale
+                f[name][2] *= scale
             st = {"nsubsteps": 1, "nsubsteps_o": 1, "dtn": 1.0, "dtnold": 1.0}
             for _ in range(3):
                 call(st, **flags)
