@@ -400,6 +400,11 @@ def density_bench(steps, size=(256, 256, 128)):
             out[mode] = {"ms_per_call": ms, "value": cells / (ms / 1e3),
                          "alg_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
                          "nsubsteps": st["nsubsteps"]}
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh).get(f"density_{np_}x{mp_}x{lp_}_f64")
+        if t:
+            out["traffic_bytes_per_call"] = t["bytes_per_call"]
     return out
 
 
