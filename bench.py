@@ -16,8 +16,15 @@ Prints one JSON line:
              u1, u2, u3, h, rhr, write x) / average step time, against the
              MI355X HBM peak of 8 TB/s; traffic = measured HBM bytes per step
              from the committed rocprofv3 counter summary, when present
-  cpu_baseline  the CPU oracle (plain C, one thread) on one step of the same
-             workload, timed on this host
+  cpu_baseline  the reference's own CPU path (its Fortran advec_dwarf compiled
+             into oracle/_ref, one thread) on a 10-step sample of the same
+             workload, timed on this host; the C oracle where that build is
+             absent
+Sub-objects: unfused_stages (the per-stage kernels), ensemble (64 uneven
+tracers in one persistent launch / HIP graph), multifield (eight fields on one
+grid), density (the density step with the velocity predictor), adjoint
+(forward+backward against one forward).  --density-only prints the density
+object alone.
 """
 import argparse
 import json
@@ -86,7 +93,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the step shards by 3-D domain decomposition, see DESIGN.md)", file=sys.stderr)
+              "(the step would shard by 3-D domain decomposition, one block per GPU)", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:
         copy_probe(a)
