@@ -16,8 +16,10 @@ from oracle import GAUGE
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "advectionhpctester_amd", "fortran", "build", "mpdata_dwarf")
-CE_EXE = os.path.join(ROOT, "advectionhpctester_amd", "fortran", "build", "mpdata_ce_check")
-RHO_EXE = os.path.join(ROOT, "advectionhpctester_amd", "fortran", "build", "mpdata_rho_check")
+FDIR = os.path.join(ROOT, "advectionhpctester_amd", "fortran")
+CE_EXE = os.path.join(FDIR, "build", "mpdata_ce_check")
+# the same programs built with -DFLOAT_PRECISION=4 (the _f32 entry points)
+BUILDS = [("build", np.float64), ("build32", np.float32)]
 GOLDEN = os.path.join(ROOT, "tests", "golden", "dwarf59_reference.txt")
 
 pytestmark = pytest.mark.gpu
@@ -98,35 +100,42 @@ def test_fortran_dwarf_standard_variant(tmp_path, orc64):
     assert np.array_equal(snaps[-1], x[1:-1, 1:-1, 1:-1])
 
 
-def test_fortran_ce_advection_equals_single_steps():
+@pytest.mark.parametrize("build", ["build", "build32"])
+def test_fortran_ce_advection_equals_single_steps(build):
     """CE_advection with th, u, v, w, tht, pstr, qv, qc (five gauge, three
-    standard fields) gives each field the bits of mpdata_step on it."""
-    if not os.path.exists(CE_EXE):
-        pytest.fail(f"{CE_EXE} is not built (make -C advectionhpctester_amd/fortran)")
-    print("fortran CE_advection check", flush=True)
-    r = subprocess.run([CE_EXE], capture_output=True, text=True, timeout=120)
+    standard fields) gives each field the bits of mpdata_step on it, in the
+    FP64 and the FP32 build of the host."""
+    exe = os.path.join(FDIR, build, "mpdata_ce_check")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (python -c 'import __graft_entry__ as g; g.build()')")
+    print(f"fortran CE_advection check ({build})", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "CE check: PASS" in r.stdout, r.stdout + r.stderr
 
 
-def test_fortran_rho_gauge_matches_oracle(tmp_path, orc64):
-    """mpdata3d_rho_gauge_gpubc through the Fortran module, three calls with
-    substepping: every array and the INOUT scalars after each call equal the
-    oracle's on the inputs the program wrote."""
+@pytest.mark.parametrize("build,npdt", BUILDS)
+def test_fortran_rho_gauge_matches_oracle(tmp_path, build, npdt):
+    """mpdata3d_rho_gauge_gpubc through the Fortran module (FP64 and FP32
+    builds), three calls with substepping: every array and the INOUT
+    scalars after each call equal the oracle's on the inputs the program
+    wrote."""
     from oracle import Oracle, shape
-    if not os.path.exists(RHO_EXE):
-        pytest.fail(f"{RHO_EXE} is not built (make -C advectionhpctester_amd/fortran)")
+    exe = os.path.join(FDIR, build, "mpdata_rho_check")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (python -c 'import __graft_entry__ as g; g.build()')")
+    orc = Oracle(npdt)
     dims = (26, 21, 12, 1)
     np_, mp, lp, ih = dims
     dump = str(tmp_path / "rho")
-    print("fortran density step check", flush=True)
-    r = subprocess.run([RHO_EXE, dump], capture_output=True, text=True, timeout=120)
+    print(f"fortran density step check ({build})", flush=True)
+    r = subprocess.run([exe, dump], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "rho check: done" in r.stdout, r.stdout + r.stderr
     states = [line.split()[3:] for line in r.stdout.splitlines() if line.startswith("call ")]
     shapes = [(3,) + shape(*dims)] * 6 + [shape(*dims, 1), shape(*dims, 2), shape(*dims, 3)] + \
         [shape(*dims)] * 8 + [(2, lp, mp), (2, lp, np_)]
 
     def load(path):
-        raw = np.fromfile(path, dtype=np.float64)
+        raw = np.fromfile(path, dtype=npdt)
         out, o = {}, 0
         for name, sh in zip(Oracle.RHO_FIELDS, shapes):
             n = int(np.prod(sh))
@@ -139,11 +148,35 @@ def test_fortran_rho_gauge_matches_oracle(tmp_path, orc64):
     got = load(dump + ".out")
     f.update(dxi=0.01, dyi=0.01, dzi=0.01, dt=40.0)
     st = dict(nsubsteps=1, nsubsteps_o=1, dtn=40.0, dtnold=40.0)
+    uint = np.uint64 if npdt == np.float64 else np.uint32
     for c in range(3):
-        st = orc64.rho_gauge(f, {"lsubstepping": 1}, st, 0.9, 80.0, 0, dims)
+        st = orc.rho_gauge(f, {"lsubstepping": 1}, st, 0.9, 80.0, 0, dims)
         assert st["status"] == 0
         assert [int(states[c][0]), int(states[c][1])] == [st["nsubsteps"], st["nsubsteps_o"]]
         assert [float(states[c][2]), float(states[c][3])] == [st["dtn"], st["dtnold"]]
     assert st["nsubsteps"] == 2
     for name in Oracle.RHO_FIELDS:
-        assert np.array_equal(got[name].view(np.uint64), f[name].view(np.uint64)), name
+        assert np.array_equal(got[name].view(uint), f[name].view(uint)), name
+
+
+def test_fortran_dwarf_fp32_build(tmp_path):
+    """The rotating-sphere dwarf through the FP32 build of the host (the
+    reference's -DFLOAT_PRECISION=4 switch, the _f32 entry points): 556 steps,
+    norms within FP32 rounding of the reference's FP64 ones."""
+    exe = os.path.join(FDIR, "build32", "mpdata_dwarf")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is not built (python -c 'import __graft_entry__ as g; g.build()')")
+    _, norms = _golden()
+    print("fortran dwarf FP32 nt=556", flush=True)
+    r = subprocess.run([exe, "556", "-", "1"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    vals = {}
+    for line in r.stdout.splitlines():
+        p = line.split()
+        if p[:2] in (["Energy", "L2"], ["Error", "L2"]):
+            vals[p[0]] = float(p[-1])
+    print("FP32 norms", vals, "FP64 reference", norms, flush=True)
+    # measured: Energy 4.065172736e-3, Error 2.810699666e-3 against the
+    # reference's printed 0.00406518 and 0.0028107 (FP64)
+    assert abs(vals["Energy"] - norms["Energy"]) < 1e-7, r.stdout
+    assert abs(vals["Error"] - norms["Error"]) < 1e-7, r.stdout
