@@ -346,17 +346,13 @@ int reduce_courants(mpdata_ctx* c, const T* ox, const T* oy, const T* oz, const 
         return fail(MPDATA_EINVAL, "ntimelevel %d (reference accepts 1 or 2)", level);
     const long long nb = Stages<T>::reduce_blocks(c->g);
     int rc;
-    if ((rc = ensure_red(c, nb * 7 * (long long)sizeof(T)))) return rc;
+    if ((rc = ensure_red(c, (nb + 1) * 7 * (long long)sizeof(T)))) return rc;
     T* part = (T*)c->red;
     HIPCHK(Stages<T>::courants(c->g, ox, oy, oz, h, level, dt * dxi, dt * dyi, dt * dzi, part, s));
-    std::vector<T> hpart((size_t)nb * 7);
-    HIPCHK(hipMemcpyAsync(hpart.data(), part, hpart.size() * sizeof(T), hipMemcpyDeviceToHost, s));
+    T m[7];
+    HIPCHK(hipMemcpyAsync(m, part + 7 * nb, sizeof m, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    for (int q = 0; q < 7; q++) {
-        T m = T(0);
-        for (long long b = 0; b < nb; b++) m = hpart[7 * b + q] > m ? hpart[7 * b + q] : m;
-        out[q] = (double)m;
-    }
+    for (int q = 0; q < 7; q++) out[q] = (double)m[q];
     return MPDATA_OK;
 }
 

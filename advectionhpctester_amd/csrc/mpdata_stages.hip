@@ -590,6 +590,28 @@ __global__ void __launch_bounds__(256) k_courants(Geo g, const T* __restrict__ o
     }
 }
 
+// Second stage of the Courant maxima: the nb per-block partials (7 each)
+// reduced by one workgroup into out[7].  A maximum does not depend on the
+// order of its operands, so the result equals a host loop over the partials.
+template <typename T>
+__global__ void __launch_bounds__(256) k_max7(const T* __restrict__ part, long long nb,
+                                              T* __restrict__ out) {
+    __shared__ T red[7][256];
+    const int t = threadIdx.x;
+    T m[7];
+    for (int q = 0; q < 7; q++) m[q] = T(0);
+    for (long long b = t; b < nb; b += 256)
+        for (int q = 0; q < 7; q++) m[q] = max2(m[q], part[7 * b + q]);
+    for (int q = 0; q < 7; q++) red[q][t] = m[q];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < 7; q++) red[q][t] = max2(red[q][t], red[q][t + s]);
+        __syncthreads();
+    }
+    if (t < 7) out[t] = red[t][0];
+}
+
 // Sums of x^2, x0^2, (x0-x)^2 and x over the interior, in double.
 template <typename T>
 __global__ void __launch_bounds__(256) k_norm_sums(Geo g, const T* __restrict__ x,
@@ -784,6 +806,9 @@ hipError_t Stages<T>::courants(const Geo& g, const T* ox, const T* oy, const T* 
                                int level, T gc1, T gc2, T gc3, T* part, hipStream_t s) {
     k_courants<T><<<cell_grid(g), cell_block(), 0, s>>>(g, ox, oy, oz, h, level, gc1, gc2, gc3,
                                                         part);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_max7<T><<<1, 256, 0, s>>>(part, reduce_blocks(g), part + 7 * reduce_blocks(g));
     return hipGetLastError();
 }
 

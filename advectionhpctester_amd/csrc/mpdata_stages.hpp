@@ -36,6 +36,8 @@ struct Stages {
     static hipError_t velprd_rotate(const Geo& g, T* u, T* v, T* w, T* ox, T* oy, T* oz,
                                     const T* h, T r, hipStream_t s);
     static long long reduce_blocks(const Geo& g);
+    // Courant maxima: per-block partials in part[0 .. 7*reduce_blocks), then
+    // the seven maxima in part[7*reduce_blocks .. +7)
     static hipError_t courants(const Geo& g, const T* ox, const T* oy, const T* oz, const T* h,
                                int level, T gc1, T gc2, T gc3, T* part, hipStream_t s);
     static hipError_t norm_sums(const Geo& g, const T* x, const T* x0, double* part,

@@ -223,3 +223,19 @@ def test_rho_gauge_full_size_properties(mp):
     assert err <= 64 * np.finfo(np.float64).eps * scale, (err, scale)
     assert torch.equal(a["rhoadv"][I], old[I])
     assert torch.equal(a["pexr2"][I], a["x_incomp"][I])
+
+
+@pytest.mark.parametrize("tdt,npdt", DTYPES)
+@pytest.mark.parametrize("level", [1, 2])
+def test_compute_courants_equals_oracle(mp, tdt, npdt, level):
+    """compute_courants (common/diagutils.f90:588-672), the driver's Courant
+    measurement: the seven maxima equal the oracle's exactly."""
+    from oracle import Oracle
+    dims = (70, 45, 30, 1)
+    f = _case(dims, 37, 0.9, npdt)
+    want = Oracle(npdt).compute_courants(f["ox"], f["oy"], f["oz"], f["h"], level, f["dxi"],
+                                         f["dyi"], f["dzi"], f["dt"], dims)
+    with mp.Domain(*dims, dtype=tdt) as d:
+        got = d.compute_courants(to_dev(f["ox"]), to_dev(f["oy"]), to_dev(f["oz"]),
+                                 to_dev(f["h"]), level, f["dxi"], f["dyi"], f["dzi"], f["dt"])
+    assert [npdt(v) for v in got] == [npdt(v) for v in want], (got, want)
