@@ -259,35 +259,45 @@ __global__ void __launch_bounds__(256)
 dim3 cell_block() { return dim3(64, 4, 1); }
 dim3 cell_grid(const Geo& g) { return dim3((g.np + 63) / 64, (g.mp + 3) / 4, g.lp); }
 
-// The density step after the predictor (:104-622 without the driver call).
-// ox(.,2) = ox(.,0)*h (:104-108) is not repeated here: the driver's level
-// rotation has just written the same products.
-template <typename T>
-int rho_core(mpdata_ctx* c, int liner, T* u1, T* u2, T* u3, T* x, T* xinc, const T* h,
-             T* rhoadv, T* rhr, const T* pexe, T* pexr1, T* pexr2, const T* bcx, const T* bcy,
-             hipStream_t s) {
-    using St = Stages<T>;
+// Argument checks of the density step, made before any GPU work (also
+// before the predictor driver of the whole routine).
+int rho_check(mpdata_ctx* c, int liner, const void* u1, const void* u2, const void* u3,
+              const void* x, const void* xinc, const void* h, const void* rhoadv, const void* rhr,
+              const void* pexe, const void* pexr1, const void* pexr2, const void* bcx,
+              const void* bcy) {
     const Geo& g = c->g;
     if (!open_all(g))
         return fail(MPDATA_EUNSUPPORTED,
                     "density step: open boundaries only (the reference's single-process path)");
     if (any_null(u1, u2, u3, x, xinc, h, rhoadv, rhr, pexe, pexr1, pexr2, bcx, bcy))
         return fail(MPDATA_EINVAL, "density step: null field");
-    {
-        const void* outs[] = {x, xinc, rhoadv, rhr, pexr1, pexr2};
-        const void* all[] = {x, xinc, rhoadv, rhr, pexr1, pexr2, h, pexe};
-        for (const void* o : outs) {
-            int same = 0;
-            for (const void* a : all) same += a == o;
-            if (same != 1) return fail(MPDATA_EINVAL, "density step: scalar fields must be distinct");
-        }
+    const void* outs[] = {x, xinc, rhoadv, rhr, pexr1, pexr2};
+    const void* all[] = {x, xinc, rhoadv, rhr, pexr1, pexr2, h, pexe};
+    for (const void* o : outs) {
+        int same = 0;
+        for (const void* a : all) same += a == o;
+        if (same != 1) return fail(MPDATA_EINVAL, "density step: scalar fields must be distinct");
     }
-    Scratch& S = c->prim;
-    int rc;
     const bool fused = !liner && c->path != MPDATA_PATH_STAGES && rho_fusable(g);
     if (c->path == MPDATA_PATH_FUSED && !fused)
         return fail(MPDATA_EUNSUPPORTED, "fused density step: liner = 0 and arrays below 2^31 "
                                          "elements only");
+    return MPDATA_OK;
+}
+
+// The density step after the predictor (:104-622 without the driver call),
+// arguments checked by rho_check.  ox(.,2) = ox(.,0)*h (:104-108) is not
+// repeated here: the driver's level rotation has just written the same
+// products.
+template <typename T>
+int rho_run(mpdata_ctx* c, int liner, T* u1, T* u2, T* u3, T* x, T* xinc, const T* h,
+            T* rhoadv, T* rhr, const T* pexe, T* pexr1, T* pexr2, const T* bcx, const T* bcy,
+            hipStream_t s) {
+    using St = Stages<T>;
+    const Geo& g = c->g;
+    Scratch& S = c->prim;
+    int rc;
+    const bool fused = !liner && c->path != MPDATA_PATH_STAGES && rho_fusable(g);
     if (fused) {
         if ((rc = ensure(c, &S.xnext, SCAL, sizeof(T)))) return rc;
         for (int d = 0; d < 3; d++)
@@ -534,8 +544,11 @@ extern "C" {
                                    void* stream) {                                               \
         int rc;                                                                                  \
         if ((rc = check_ctx(ctx, RB))) return rc;                                                \
-        return rho_core<T>(ctx, liner, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1,  \
-                           pexr2, bcx, bcy, (hipStream_t)stream);                                \
+        if ((rc = rho_check(ctx, liner, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1, \
+                            pexr2, bcx, bcy)))                                                   \
+            return rc;                                                                           \
+        return rho_run<T>(ctx, liner, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1,   \
+                          pexr2, bcx, bcy, (hipStream_t)stream);                                 \
     }                                                                                            \
     int mpdata3d_rho_gauge_gpubc##SFX(                                                           \
         mpdata_ctx* ctx, int lupdatemulti, int* nsubsteps, int* nsubsteps_o, int lsubstepping,   \
@@ -548,17 +561,16 @@ extern "C" {
         int rc;                                                                                  \
         if ((rc = check_ctx(ctx, RB))) return rc;                                                \
         if ((rc = check_dims(ctx, ipoles0, ibcx0, ibcy0, ibcz0, np, mp, lp, ih))) return rc;     \
-        if (!open_all(ctx->g))                                                                   \
-            return fail(MPDATA_EUNSUPPORTED, "density step: open boundaries only");              \
-        if (any_null(x_in, x_incomp, rhoadv, rhr, pexe, pexr1, pexr2, bcx, bcy))                 \
-            return fail(MPDATA_EINVAL, "density step: null field");                             \
+        if ((rc = rho_check(ctx, liner, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1, \
+                            pexr2, bcx, bcy)))                                                   \
+            return rc;                                                                           \
         if ((rc = driver_impl<T>(ctx, nsubsteps, nsubsteps_o, lsubstepping, dtn, dtnold,         \
                                  ltimeadapt, lvertsplit, lcrdiag_nphlf, crmaxlim, dtmax, ox, oy, \
                                  oz, u, v, w, u1, u2, u3, h, dxi, dyi, dzi, dt,                  \
                                  (hipStream_t)stream)))                                          \
             return rc;                                                                           \
-        return rho_core<T>(ctx, liner, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1,  \
-                           pexr2, bcx, bcy, (hipStream_t)stream);                                \
+        return rho_run<T>(ctx, liner, u1, u2, u3, x_in, x_incomp, h, rhoadv, rhr, pexe, pexr1,   \
+                          pexr2, bcx, bcy, (hipStream_t)stream);                                 \
     }
 
 DEFINE_RHO_API(double, _f64, 8)

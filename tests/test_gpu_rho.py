@@ -165,6 +165,15 @@ def test_rho_gauge_errors(mp):
             d.rho_gauge_core(g["u1"], g["u2"], g["u3"], g["x_in"], g["x_in"], g["h"],
                              g["rhoadv"], g["rhr"], g["pexe"], g["pexr1"], g["pexr2"], g["bcx"],
                              g["bcy"])
+        # the whole routine rejects it before the predictor touches anything
+        keep = {k: g[k].clone() for k in ("ox", "u", "u1")}
+        s0 = dict(st)
+        with pytest.raises(MpdataError):
+            d.rho_gauge_gpubc(s0, *args, g["x_in"], g["x_in"], *rest)
+        torch.cuda.synchronize()
+        for k, v in keep.items():
+            assert torch.equal(g[k], v), f"{k} changed by a rejected call"
+        assert s0 == st
         big = dict(st, nsubsteps=2)
         f2 = _case(dims, 3, 25.0, np.float64)
         g2 = {k: to_dev(v) for k, v in f2.items() if isinstance(v, np.ndarray)}
