@@ -244,9 +244,11 @@ int mpdata_velprd_traj0_adapt_dt_f32(mpdata_ctx *ctx, float *ox, float *oy, floa
  * or adaptivity the Courant read-backs synchronise the stream; with a fixed
  * step (neither) the Courant number, which the reference only prints, is
  * not measured and the call is asynchronous.
- * MPDATA_ESTOP where the reference STOPs (substepping with adaptivity;
- * Courant/nsubsteps > 10 while substepping) and where its loops would not
- * end (no admissible step within 1000 trials). */
+ * MPDATA_ESTOP where the reference STOPs (substepping with adaptivity,
+ * before any GPU work; Courant/nsubsteps > 10 while substepping) and where
+ * its loops would not end (no admissible step within 1000 trials); the last
+ * two are found after the Courant measurement, so level 1 of ox, oy, oz
+ * and uadv..wadv may then hold a trial prediction. */
 int mpdata_velprd_driver_f64(mpdata_ctx *ctx, int *nsubsteps, int *nsubsteps_o,
                              int lsubstepping, double *dtn, double *dtnold, int ltimeadapt,
                              int lvertsplit, int lcrdiag_nphlf, double crmaxlim, double dtmax,
