@@ -964,6 +964,19 @@ int fused_chunk_env() {
     return c >= 1 ? c : -1;
 }
 
+// Workgroups of a persistent launch.  The kernels deal workgroups to the
+// NXCD XCDs round-robin and give XCD x the items of its contiguous share
+// (items_per_xcd of them), so the grid is a multiple of NXCD with no more
+// workgroups per XCD than that share has items (a smaller share would
+// leave slots idle while items of the XCD wait for a second round); fewer
+// than NXCD items run without the split.
+int persistent_grid(long long nitem, long long items_per_xcd, int ncu, int per_cu) {
+    if (nitem < NXCD) return (int)std::max(1LL, nitem);
+    const long long slots = (long long)ncu * std::max(per_cu, 1);
+    const long long per_xcd = std::max(1LL, std::min(items_per_xcd, slots / NXCD));
+    return (int)(per_xcd * NXCD);
+}
+
 template <int RX, int RY>
 long long items_of(const Geo& g, int clen) {
     const long long ntx = (g.np + RX - 5) / (RX - 4), nty = (g.mp + RY - 5) / (RY - 4);
@@ -1000,7 +1013,9 @@ int per_cu_of() {
 // six, with 64).  A smaller domain takes 32x16 tiles (two workgroups per
 // CU) and the longest chunk of 64 or 32 planes that gives every workgroup
 // slot two items, else 16 (measured: on smaller domains more, shorter
-// items beat fewer, longer ones).
+// items beat fewer, longer ones; a rounds model over 64..8 planes chose
+// 64 at 256x256x128 and was 7 % slower there), and 8 or 4 planes when the
+// domain is so small that its items still fit one per CU.
 template <typename T>
 void choose_tiling(const Geo& g, int& cfg, int& clen) {
     int dev = 0, ncu = 0;
@@ -1037,6 +1052,17 @@ void choose_tiling(const Geo& g, int& cfg, int& clen) {
             for (int c : {64, 32}) {
                 if (items_of<32, 16>(g, c) >= 4LL * ncu) { clen = c; break; }
             }
+        // A small domain whose items all get a CU of their own marches
+        // clen + kRamp planes in one round: shorter chunks finish sooner as
+        // long as every XCD's share still fits one workgroup per CU
+        // (measured: 32^3 68 -> 44 us, 59^3 79 -> 45 us per step).
+        if (env_clen <= 0 && clen == 16) {
+            const long long tiles_per_xcd = (items_of<32, 16>(g, g.lp) + NXCD - 1) / NXCD;
+            for (int c : {8, 4}) {
+                if (tiles_per_xcd * ((g.lp + c - 1) / c) > std::max(1, ncu / NXCD)) break;
+                clen = c;
+            }
+        }
     }
     if (clen > g.lp) clen = g.lp;
 }
@@ -1057,7 +1083,8 @@ hipError_t launch2(const Geo& g, const Off& o, int clen, const T* x, T* xout, co
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_gauge2<T, RX, RY, MINB>,
                                                          RX * RY, 0);
     if (e != hipSuccess) return e;
-    const int grid = std::max(1, std::min(nitem, ncu * std::max(per_cu, 1)));
+    const int grid = persistent_grid(nitem, (long long)(ntx * nty + NXCD - 1) / NXCD * nchunk, ncu,
+                                     per_cu);
     k_fused_gauge2<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>(g, o, ntx, nty, nchunk, clen, x, u1,
                                                              u2, u3, rhr, h, bcx, bcy, xout);
     return hipGetLastError();
@@ -1104,7 +1131,8 @@ hipError_t launch_dens(const Geo& g, const Off& o, int clen, const T* x, T* xout
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_dens<T, RX, RY, MINB>,
                                                          RX * RY, 0);
     if (e != hipSuccess) return e;
-    const int grid = std::max(1, std::min(nitem, ncu * std::max(per_cu, 1)));
+    const int grid = persistent_grid(nitem, (long long)(ntx * nty + NXCD - 1) / NXCD * nchunk, ncu,
+                                     per_cu);
     k_fused_dens<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>(g, o, ntx, nty, nchunk, clen, x, u1, u2,
                                                            u3, h, bcx, bcy, xout, d);
     return hipGetLastError();
@@ -1229,7 +1257,7 @@ hipError_t launch_batch(const void* table, int n, int nitem, int shared, hipStre
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, k_fused_gauge2_batch<T, RX, RY, MINB>, RX * RY, 0);
     if (e != hipSuccess) return e;
-    const int grid = std::max(1, std::min(nitem, ncu * std::max(per_cu, 1)));
+    const int grid = persistent_grid(nitem, ((long long)nitem + NXCD - 1) / NXCD, ncu, per_cu);
     k_fused_gauge2_batch<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>((const Member<T>*)table, n, nitem,
                                                                    shared);
     return hipGetLastError();
