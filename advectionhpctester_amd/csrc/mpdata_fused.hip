@@ -22,6 +22,7 @@
 #include "mpdata_fused.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <cstdlib>
 
 namespace mpdata {
@@ -63,22 +64,6 @@ __device__ __forceinline__ T ldo(const T* p, uint32_t off) {
 template <typename T>
 __device__ __forceinline__ void sto(T* p, uint32_t off, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off) = v;
-}
-
-// A register copy made here rather than where the loop's phi nodes would
-// put it (the end of the iteration).  Shifting a register queue with these
-// before the new plane is loaded frees the slot the load writes, so the
-// load lands in the register the next iteration reads and no end-of-step
-// copy has to wait for it.
-__device__ __forceinline__ double qcopy(double v) {
-    double r;
-    asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "v"(v));
-    return r;
-}
-__device__ __forceinline__ float qcopy(float v) {
-    float r;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
-    return r;
 }
 
 // The value held by lane l-1 of the wave (DPP wave_shr:1; lane 0 gets 0).
@@ -438,11 +423,6 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     auto& sx = L.sx;
     auto& sxa = L.sxa;
     auto& sv1 = L.sv1;
-    auto& sv2 = L.sv2;
-    auto& scp = L.scp;
-    auto& scn = L.scn;
-    auto& sf1 = L.sf1;
-    auto& sf2 = L.sf2;
     const int np = g.np, mp = g.mp, lp = g.lp;
     const int t = threadIdx.x;
     const int pi = t % RX, pj = t / RX;  // region index
@@ -505,33 +485,76 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     //   xq1..3 = x(K-1..K+1), u1a0/1 = u1(i,K-1..K), u1b0/1 = u1(i+1,K-1..K),
     //   u2a/u2b likewise at j and j+1, u3q0..2 = faces K-1..K+1,
     //   hq1..2 = h(K-1..K), rq2 = rhr(K), v3q1..2 = V3 faces K-2..K-1,
-    //   xmx0/xmn0 = bounds of x for the limiter on plane K-1.
+    //   xmx0/xmn0 = bounds of x for the limiter on plane K-1 (xmx1/xmn1:
+    //   plane K, formed in A).
     // The column's own xant of planes K-2..K comes from sxa: the slot of
     // an own position is rewritten only by this thread, later in program
     // order.
-    T xq1 = ldo(pl(x, SXY, kc(Ks - 1)), so), xq2 = ldo(pl(x, SXY, kc(Ks)), so);
-    T xq3 = ldo(pl(x, SXY, kc(Ks + 1)), so);
-    T u1a0 = ldo(pl(u1, FXY, kc(Ks - 1)), fo), u1a1 = ldo(pl(u1, FXY, kc(Ks)), fo);
-    T u1b0 = ldo(pl(u1, FXY, kc(Ks - 1)) + 1, fo), u1b1 = ldo(pl(u1, FXY, kc(Ks)) + 1, fo);
-    T u2a0 = ldo(pl(u2, YXY, kc(Ks - 1)), so), u2a1 = ldo(pl(u2, YXY, kc(Ks)), so);
-    T u2b0 = ldo(pl(u2, YXY, kc(Ks - 1)) + o.yx, so), u2b1 = ldo(pl(u2, YXY, kc(Ks)) + o.yx, so);
-    T u3q0 = ldo(pl(u3, SXY, fc(Ks - 1)), so), u3q1 = ldo(pl(u3, SXY, fc(Ks)), so);
-    T u3q2 = ldo(pl(u3, SXY, fc(Ks + 1)), so);
-    T hq1 = ldo(pl(h, SXY, kc(Ks - 1)), so), hq2 = ldo(pl(h, SXY, kc(Ks)), so);
+    // The k loop is unrolled six-fold: step PH of a round handles a plane
+    // K with K - Ks = PH (mod 6), so every ring slot (K - Ks mod 2 or 3) is
+    // a compile-time constant, which makes the LDS offsets immediates, and
+    // the queues are arrays whose roles rotate with PH instead of being
+    // copied: a new plane is loaded into the slot of the oldest one.
+    T xq[3], u1a[2], u1b[2], u2a[2], u2b[2], u3q[3], hq[2];
+    xq[0] = ldo(pl(x, SXY, kc(Ks - 1)), so); xq[1] = ldo(pl(x, SXY, kc(Ks)), so);
+    xq[2] = ldo(pl(x, SXY, kc(Ks + 1)), so);
+    u1a[0] = ldo(pl(u1, FXY, kc(Ks - 1)), fo); u1a[1] = ldo(pl(u1, FXY, kc(Ks)), fo);
+    u1b[0] = ldo(pl(u1, FXY, kc(Ks - 1)) + 1, fo); u1b[1] = ldo(pl(u1, FXY, kc(Ks)) + 1, fo);
+    u2a[0] = ldo(pl(u2, YXY, kc(Ks - 1)), so); u2a[1] = ldo(pl(u2, YXY, kc(Ks)), so);
+    u2b[0] = ldo(pl(u2, YXY, kc(Ks - 1)) + o.yx, so); u2b[1] = ldo(pl(u2, YXY, kc(Ks)) + o.yx, so);
+    u3q[0] = ldo(pl(u3, SXY, fc(Ks - 1)), so); u3q[1] = ldo(pl(u3, SXY, fc(Ks)), so);
+    u3q[2] = ldo(pl(u3, SXY, fc(Ks + 1)), so);
+    hq[0] = ldo(pl(h, SXY, kc(Ks - 1)), so); hq[1] = ldo(pl(h, SXY, kc(Ks)), so);
     T rq2 = T(0);
     if constexpr (!DENS) rq2 = ldo(pl(rhr, SXY, kc(Ks)), so);
-    T v3q1 = T(0), v3q2 = T(0);
-    T xmx0 = T(0), xmn0 = T(0), xmx1 = T(0), xmn1 = T(0);  // x in-plane max/min, planes K-1, K
+    T v3q[2] = {T(0), T(0)};
+    T xmx[2] = {T(0), T(0)}, xmn[2] = {T(0), T(0)};
     T f3c = T(0);  // limited flux through the upper z-face of the last output plane
 
     // x of the first plane into LDS; the barrier also orders this item
     // after every LDS read of the previous one
-    sx[Ks & 1][pj + 1][pi + 1] = in_dom ? xq2 : ld(own, Ks);
-    if (t < NRING) sx[Ks & 1][rsj][rsi] = ld(ring, Ks);
+    sx[0][pj + 1][pi + 1] = in_dom ? xq[1] : ld(own, Ks);
+    if (t < NRING) sx[0][rsj][rsi] = ld(ring, Ks);
     __syncthreads();
 
+    // The tile arrays from sv2 on lie beyond the 64 KiB that a ds
+    // instruction's immediate offset reaches from the lane's position.  They
+    // are addressed from two per-lane byte offsets made opaque here, so that
+    // every access is one of them plus an immediate; otherwise the compiler
+    // keeps one address register per array and ring slot through the loop.
+    using LT = FusedLds<T, RX, RY>;
+    constexpr int SZ = (int)sizeof(T), PLN = RY * RX * SZ;  // one plane of a tile array
+    constexpr int MID = (int)offsetof(LT, sv2);
+    constexpr int HI = (int)offsetof(LT, e_h) - RX * SZ;  // e arrays are read one row back
+    uint32_t lmid = uint32_t(MID + t * SZ), lhi = uint32_t(HI + t * SZ);
+    asm volatile("" : "+v"(lmid), "+v"(lhi));
+    char* const lds0 = reinterpret_cast<char*>(&L);
+    // array `arr` at ring slot s, offset (db rows, da columns) from the lane
+#define LMID(arr, s, db, da) \
+    (*reinterpret_cast<T*>(lds0 + lmid + ((int)offsetof(LT, arr) - MID + (s) * PLN + ((db) * RX + (da)) * SZ)))
+#define LHI(arr, r) (*reinterpret_cast<T*>(lds0 + lhi + ((int)offsetof(LT, arr) - HI + (r) * SZ)))
+
     uint32_t so_k = so, fo_k = fo;
-    for (int K = Ks; K <= Ke; ++K) {
+    auto step = [&](auto phase, int K) __attribute__((always_inline)) {
+        constexpr int PH = decltype(phase)::value;
+        // K opaque: every plane-dependent quantity is formed within the
+        // step, rather than as one induction variable per unrolled step
+        // held in registers through the loop
+        asm volatile("" : "+s"(K));
+        // queue slots: oldest, middle, newest of a three-plane queue; older,
+        // newer of a two-plane one
+        constexpr int Q3a = PH % 3, Q3b = (PH + 1) % 3, Q3c = (PH + 2) % 3;
+        constexpr int Q2a = PH % 2, Q2b = (PH + 1) % 2;
+        // LDS ring slots of plane K-d: mod 3 (R0, R1, R2 for d = 0, 1, 2) and
+        // mod 2 (E0 for even d, E1 for odd d)
+        constexpr int R0 = Q3a, R1 = Q3c, R2 = Q3b, E0 = Q2a, E1 = Q2b;
+        T& xq1 = xq[Q3a]; T& xq2 = xq[Q3b]; T& xq3 = xq[Q3c];
+        T& u1a0 = u1a[Q2a]; T& u1a1 = u1a[Q2b]; T& u1b0 = u1b[Q2a]; T& u1b1 = u1b[Q2b];
+        T& u2a0 = u2a[Q2a]; T& u2a1 = u2a[Q2b]; T& u2b0 = u2b[Q2a]; T& u2b1 = u2b[Q2b];
+        T& u3q0 = u3q[Q3a]; T& u3q1 = u3q[Q3b]; T& u3q2 = u3q[Q3c];
+        T& hq1 = hq[Q2a]; T& hq2 = hq[Q2b];
+        T& v3q1 = v3q[Q2a]; T& v3q2 = v3q[Q2b];
+        T& xmx0 = xmx[Q2a]; T& xmn0 = xmn[Q2a]; T& xmx1 = xmx[Q2b]; T& xmn1 = xmn[Q2b];
         // the column offsets are opaque to the optimiser at each plane, so
         // loads keep the saddr form (plane pointer + offset) instead of
         // becoming per-lane 64-bit pointer inductions
@@ -569,11 +592,10 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         T nv2_h = T(0), nv2_u1a = T(0), nv2_u1b = T(0), nv2_u3a = T(0), nv2_u3b = T(0);
         if constexpr (XJ<RX, RY>::value) {
             if (BC_on) {
-                const int e = pj * RX + pi;
-                L.e_h[e] = hq1;
-                L.e_u1[e] = u1a0;
-                L.e_u3a[e] = u3q0;
-                L.e_u3b[e] = u3q1;
+                LHI(e_h, 0) = hq1;  // element e = t
+                LHI(e_u1, 0) = u1a0;
+                LHI(e_u3a, 0) = u3q0;
+                LHI(e_u3b, 0) = u3q1;
             }
         } else {
             nv2_h = ldo(pl(h, SXY, Pc) - o.sx, so);
@@ -586,7 +608,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         // ---- A: xant on plane K ------------------------------------------
         T xa_new = T(0);
         if (A_on) {
-            auto& sx = L.sx[K & 1];
+            auto& sx = L.sx[E0];
             const bool gnd = K == 1, sky = K == lp;
             T hinv = T(1) / hq2;
             T xc = xq2;
@@ -607,7 +629,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             }
             if constexpr (DENS) xa_new = (xc - (f1p - f1 + f2p - f2 + f3p - f3) / hq2);
             else xa_new = rq2 * (xc - (f1p - f1 + f2p - f2 + f3p - f3) * hinv);
-            sxa[K % 3][pj][pi] = xa_new;
+            sxa[R0][pj][pi] = xa_new;
             // bounds of x for the limiter on plane K: the 7-point stencil,
             // where out-of-domain neighbours are edge copies of the centre
             // (cp_last_to_halo_xy) and leave max/min unchanged
@@ -622,24 +644,24 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             xmn1 = n;
         }
         if (An_on) {
-            sx[(K + 1) & 1][pj + 1][pi + 1] = in_dom ? xq3 : xown;
-            if (t < NRING) sx[(K + 1) & 1][rsj][rsi] = xr;
+            sx[E1][pj + 1][pi + 1] = in_dom ? xq3 : xown;
+            if (t < NRING) sx[E1][rsj][rsi] = xr;
         }
         __syncthreads();
-#define XA(pl, a, b) sxa[(pl) % 3][(b) + 2][(a) + 2]
-#define CP(pl, a, b) scp[(pl) & 1][(b) + 2][(a) + 2]
-#define CN(pl, a, b) scn[(pl) & 1][(b) + 2][(a) + 2]
+// xant, cp, cn of plane K-d at tile position (a, b)
+#define XA(d, a, b) sxa[(d) == 0 ? R0 : ((d) == 1 ? R1 : R2)][(b) + 2][(a) + 2]
+#define CP(d, a, b) LMID(scp, ((d) & 1) ? E1 : E0, (b) - lj, (a) - li)
+#define CN(d, a, b) LMID(scn, ((d) & 1) ? E1 : E0, (b) - lj, (a) - li)
         if constexpr (XJ<RX, RY>::value) {
             if (BC_on && do_v2) {  // pj >= 1 where V2 is formed
-                const int e = (pj - 1) * RX + pi;
-                nv2_h = L.e_h[e];
-                nv2_u1a = L.e_u1[e];
-                nv2_u3a = L.e_u3a[e];
-                nv2_u3b = L.e_u3b[e];
+                nv2_h = LHI(e_h, -RX);  // element e = t - RX
+                nv2_u1a = LHI(e_u1, -RX);
+                nv2_u3a = LHI(e_u3a, -RX);
+                nv2_u3b = LHI(e_u3b, -RX);
                 // u1 on face i+1 of column j-1: the next position's face i,
                 // except on the last column, whose right neighbour is a
                 // mirrored ring position
-                if (gi < np) nv2_u1b = L.e_u1[e + 1];
+                if (gi < np) nv2_u1b = LHI(e_u1, -RX + 1);
                 else nv2_u1b = ldo(pl(u1, FXY, Pc) - o.fx + 1, fo);
             }
         }
@@ -648,18 +670,18 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         // ---- B: pair sums, V1/V2 on plane P=K-1, V3 on face K --------------
         T s1n = T(0), s2n = T(0), v3n = T(0);
         if (A_on) {
-            if (do_v1) s1n = xa_new + XA(K, li - 1, lj);
-            if (do_v2) s2n = xa_new + XA(K, li, lj - 1);
+            if (do_v1) s1n = xa_new + XA(0, li - 1, lj);
+            if (do_v2) s2n = xa_new + XA(0, li, lj - 1);
             if (do_c && K >= 2 && K >= k0 - 1) {
                 T hmz = T(1) / (T(0.5) * (hq1 + hq2));
                 T s1 = u1a0 + u1a1 + u1b1 + u1b0;
                 T s2 = u2a0 + u2b0 + u2b1 + u2a1;
-                T cr = s1 * rat4(XA(K - 1, li - 1, lj), XA(K, li - 1, lj), XA(K - 1, li + 1, lj),
-                                 XA(K, li + 1, lj)) +
-                       s2 * rat4(XA(K - 1, li, lj - 1), XA(K, li, lj - 1), XA(K - 1, li, lj + 1),
-                                 XA(K, li, lj + 1));
+                T cr = s1 * rat4(XA(1, li - 1, lj), XA(0, li - 1, lj), XA(1, li + 1, lj),
+                                 XA(0, li + 1, lj)) +
+                       s2 * rat4(XA(1, li, lj - 1), XA(0, li, lj - 1), XA(1, li, lj + 1),
+                                 XA(0, li, lj + 1));
                 T a = u3q1;
-                v3n = vdyf(XA(K - 1, li, lj), xa_new, a, hmz) - T(0.125) * a * hmz * (cr);
+                v3n = vdyf(XA(1, li, lj), xa_new, a, hmz) - T(0.125) * a * hmz * (cr);
             }
         }
         if (BC_on) {
@@ -668,39 +690,39 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             if (do_v1) {
                 T hmx = T(1) / (T(0.5) * (nv1_h + hq1));
                 T s2 = nv1_u2a + nv1_u2b + u2b0 + u2a0;
-                T cr = s2 * rat4(XA(P, li - 1, lj - 1), XA(P, li, lj - 1), XA(P, li - 1, lj + 1),
-                                 XA(P, li, lj + 1));
+                T cr = s2 * rat4(XA(1, li - 1, lj - 1), XA(1, li, lj - 1), XA(1, li - 1, lj + 1),
+                                 XA(1, li, lj + 1));
                 if (zterms) {
                     T s3 = nv1_u3a + nv1_u3b + u3q1 + u3q0;
-                    cr = cr + s3 * ((s1n - (XA(K - 2, li, lj) + XA(K - 2, li - 1, lj))) * T(0.25));
+                    cr = cr + s3 * ((s1n - (XA(2, li, lj) + XA(2, li - 1, lj))) * T(0.25));
                 }
                 T a = u1a0;
-                val1 = vdyf(XA(P, li - 1, lj), XA(P, li, lj), a, hmx) - T(0.125) * a * hmx * (cr);
+                val1 = vdyf(XA(1, li - 1, lj), XA(1, li, lj), a, hmx) - T(0.125) * a * hmx * (cr);
             }
             if (do_v2) {
                 T hmy = T(1) / (T(0.5) * (nv2_h + hq1));
                 T s1 = nv2_u1a + u1a0 + u1b0 + nv2_u1b;
-                T cr = s1 * rat4(XA(P, li - 1, lj - 1), XA(P, li - 1, lj), XA(P, li + 1, lj - 1),
-                                 XA(P, li + 1, lj));
+                T cr = s1 * rat4(XA(1, li - 1, lj - 1), XA(1, li - 1, lj), XA(1, li + 1, lj - 1),
+                                 XA(1, li + 1, lj));
                 if (zterms) {
                     T s3 = nv2_u3a + u3q0 + u3q1 + nv2_u3b;
-                    cr = cr + s3 * ((s2n - (XA(K - 2, li, lj) + XA(K - 2, li, lj - 1))) * T(0.25));
+                    cr = cr + s3 * ((s2n - (XA(2, li, lj) + XA(2, li, lj - 1))) * T(0.25));
                 }
                 T a = u2a0;
-                val2 = vdyf(XA(P, li, lj - 1), XA(P, li, lj), a, hmy) - T(0.125) * a * hmy * (cr);
+                val2 = vdyf(XA(1, li, lj - 1), XA(1, li, lj), a, hmy) - T(0.125) * a * hmy * (cr);
             }
-            sv1[P & 1][pj][pi] = val1;
-            sv2[P & 1][pj][pi] = val2;
+            sv1[E1][pj][pi] = val1;
+            LMID(sv2, E1, 0, 0) = val2;
         }
         // ---- limited fluxes through the left x-face and lower y-face of this
         //      position on plane Q (V and cp/cn of Q are from the previous
         //      step); stage D reads them for both faces a cell shares, so
         //      each face flux is formed once
         if (Dq && do_f) {
-            sf1[pj][pi] = gflux(sv1[Q & 1][pj][pi], CP(Q, li, lj), CN(Q, li - 1, lj),
-                                CP(Q, li - 1, lj), CN(Q, li, lj));
-            sf2[pj][pi] = gflux(sv2[Q & 1][pj][pi], CP(Q, li, lj), CN(Q, li, lj - 1),
-                                CP(Q, li, lj - 1), CN(Q, li, lj));
+            LMID(sf1, 0, 0, 0) = gflux(sv1[E0][pj][pi], CP(2, li, lj), CN(2, li - 1, lj),
+                                CP(2, li - 1, lj), CN(2, li, lj));
+            LMID(sf2, 0, 0, 0) = gflux(LMID(sv2, E0, 0, 0), CP(2, li, lj), CN(2, li, lj - 1),
+                                CP(2, li, lj - 1), CN(2, li, lj));
         }
         // The loaded source-column queues are not read after this point:
         // rotate them to plane K+1 and load the new planes into the freed
@@ -708,18 +730,15 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         // no copy at the end of the step waits for it; its latency is
         // covered by C, D and the next step's barrier.
         {
+            // the oldest slot of each queue becomes the newest of step K+1
             const int Kn = K + 1;
-            xq1 = qcopy(xq2); xq2 = qcopy(xq3);
-            u1a0 = qcopy(u1a1); u1b0 = qcopy(u1b1); u2a0 = qcopy(u2a1); u2b0 = qcopy(u2b1);
-            u3q0 = qcopy(u3q1); u3q1 = qcopy(u3q2);
-            hq1 = qcopy(hq2);
-            xq3 = ldo(pl(x, SXY, kc(Kn + 1)), so);
-            u1a1 = ldo(pl(u1, FXY, kc(Kn)), fo);
-            u1b1 = ldo(pl(u1, FXY, kc(Kn)) + 1, fo);
-            u2a1 = ldo(pl(u2, YXY, kc(Kn)), so);
-            u2b1 = ldo(pl(u2, YXY, kc(Kn)) + o.yx, so);
-            u3q2 = ldo(pl(u3, SXY, fc(Kn + 1)), so);
-            hq2 = ldo(pl(h, SXY, kc(Kn)), so);
+            xq1 = ldo(pl(x, SXY, kc(Kn + 1)), so);
+            u1a0 = ldo(pl(u1, FXY, kc(Kn)), fo);
+            u1b0 = ldo(pl(u1, FXY, kc(Kn)) + 1, fo);
+            u2a0 = ldo(pl(u2, YXY, kc(Kn)), so);
+            u2b0 = ldo(pl(u2, YXY, kc(Kn)) + o.yx, so);
+            u3q0 = ldo(pl(u3, SXY, fc(Kn + 1)), so);
+            hq1 = ldo(pl(h, SXY, kc(Kn)), so);
             if constexpr (!DENS) rq2 = ldo(pl(rhr, SXY, kc(Kn)), so);
         }
         __syncthreads();
@@ -736,18 +755,18 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             T rp = T(0), rn = T(0);
             if (do_c) {
                 T mxo = xmx0, mno = xmn0;
-                const T xaP = XA(P, li, lj);
-                T mx = XA(P, li - 1, lj), mn = mx;
+                const T xaP = XA(1, li, lj);
+                T mx = XA(1, li - 1, lj), mn = mx;
 #define ACC(v, lo, hi) { T t_ = (v); lo = min2(lo, t_); hi = max2(hi, t_); }
-                ACC(xaP, mn, mx) ACC(XA(P, li + 1, lj), mn, mx)
-                ACC(XA(P, li, lj - 1), mn, mx) ACC(XA(P, li, lj + 1), mn, mx)
-                if (!gnd) ACC(XA(P - 1, li, lj), mn, mx)
+                ACC(xaP, mn, mx) ACC(XA(1, li + 1, lj), mn, mx)
+                ACC(XA(1, li, lj - 1), mn, mx) ACC(XA(1, li, lj + 1), mn, mx)
+                if (!gnd) ACC(XA(2, li, lj), mn, mx)
                 if (!sky) ACC(xa_new, mn, mx)
 #undef ACC
                 mx = max2(mx, mxo);
                 mn = min2(mn, mno);
-                T a1p = sv1[P & 1][pj][pi + 1], a1 = sv1[P & 1][pj][pi];
-                T a2p = sv2[P & 1][pj + 1][pi], a2 = sv2[P & 1][pj][pi];
+                T a1p = sv1[E1][pj][pi + 1], a1 = sv1[E1][pj][pi];
+                T a2p = LMID(sv2, E1, 1, 0), a2 = LMID(sv2, E1, 0, 0);
                 T a3p, a3;
                 if (gnd) { a3p = v3n; a3 = -a3p; }
                 else if (sky) { a3 = v3q2; a3p = -a3; }
@@ -756,15 +775,15 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 rp = (mx - xaP) * hc / (pn(a1p) + pp(a1) + pn(a2p) + pp(a2) + pn(a3p) + pp(a3) + ep);
                 rn = (xaP - mn) * hc / (pp(a1p) + pn(a1) + pp(a2p) + pn(a2) + pp(a3p) + pn(a3) + ep);
             }
-            scp[P & 1][pj][pi] = rp;
-            scn[P & 1][pj][pi] = rn;
+            LMID(scp, E1, 0, 0) = rp;
+            LMID(scn, E1, 0, 0) = rn;
         }
         // the lower z-face flux of the first output plane of the range is
         // the upper face of plane k0-1, formed here on the step before (the
         // two-plane cp/cn ring no longer holds plane Q-1 once plane Q is
         // output); later planes carry it from the previous plane
         if (Q == k0 - 1 && Q >= 1 && do_d)
-            f3c = gflux(v3q2, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
+            f3c = gflux(v3q2, CP(1, li, lj), CN(2, li, lj), CP(2, li, lj), CN(1, li, lj));
         // ---- D: the new x on plane Q=K-2 (neighbour data of plane Q was
         //      written in the previous step; own cp/cn of Q+1 just now) ----
         if (D_on) {
@@ -772,22 +791,22 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             // x/y faces from stage B of this step; the lower z-face flux is
             // the upper one of plane Q-1 (the same expression).  On the sky
             // plane C did not run this step, so the ring still holds Q-1.
-            const T F1 = sf1[pj][pi], F1p = sf1[pj][pi + 1];
-            const T F2 = sf2[pj][pi], F2p = sf2[pj + 1][pi];
+            const T F1 = LMID(sf1, 0, 0, 0), F1p = LMID(sf1, 0, 0, 1);
+            const T F2 = LMID(sf2, 0, 0, 0), F2p = LMID(sf2, 0, 1, 0);
             T F3, F3p;
             if (gnd) {
                 const T v = v3q2;
-                F3 = gflux(-v, CP(Q, li, lj), CN(Q + 1, li, lj), CP(Q + 1, li, lj), CN(Q, li, lj));
-                F3p = gflux(v, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
+                F3 = gflux(-v, CP(2, li, lj), CN(1, li, lj), CP(1, li, lj), CN(2, li, lj));
+                F3p = gflux(v, CP(1, li, lj), CN(2, li, lj), CP(2, li, lj), CN(1, li, lj));
             } else {
                 F3 = f3c;
-                if (sky) F3p = gflux(-v3q1, CP(Q - 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q - 1, li, lj));
-                else F3p = gflux(v3q2, CP(Q + 1, li, lj), CN(Q, li, lj), CP(Q, li, lj), CN(Q + 1, li, lj));
+                if (sky) F3p = gflux(-v3q1, CP(3, li, lj), CN(2, li, lj), CP(2, li, lj), CN(3, li, lj));
+                else F3p = gflux(v3q2, CP(1, li, lj), CN(2, li, lj), CP(2, li, lj), CN(1, li, lj));
             }
             f3c = F3p;
             T* const xoq = pl(xo, SXY, Q);
             if constexpr (DENS) {
-                const T xnew = (XA(Q, li, lj) - (F1p - F1 + F2p - F2 + F3p - F3) / hQ);
+                const T xnew = (XA(2, li, lj) - (F1p - F1 + F2p - F2 + F3p - F3) / hQ);
                 sto(xoq, so, xnew);
                 const T xi = xnew * hQ;
                 T* const xincq = pl(dio->xinc, SXY, Q);
@@ -799,7 +818,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 sto(pl(dio->rhr, SXY, Q), so, rho_n / xi);
             } else {
                 const T r = rQ;
-                sto(xoq, so, r * (XA(Q, li, lj) / r - (F1p - F1 + F2p - F2 + F3p - F3) / hQ));
+                sto(xoq, so, r * (XA(2, li, lj) / r - (F1p - F1 + F2p - F2 + F3p - F3) / hQ));
             }
             if (gi == 1 || gi == np || gj == 1 || gj == mp) {
                 const T xe = ldo(pl(x, SXY, Q), so);
@@ -831,33 +850,49 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 sto(pl(dio->m3, SXY, Q), so, f3ns + F3);
                 if (gi == np) {
                     const T w = mpdata::donor(xc, bcx[(gj - 1) + mp * ((Q - 1) + lp)], ldo(u1q + 1, fo));
-                    const T a = sv1[Q & 1][pj][pi + 1];
+                    const T a = sv1[E0][pj][pi + 1];
                     sto(pl(dio->m1, FXY, Q) + 1, fo,
-                        w + mpdata::pp(a) * mpdata::min3(T(1), CP(Q, li + 1, lj), CN(Q, li, lj)) -
-                            mpdata::pn(a) * mpdata::min3(T(1), CP(Q, li, lj), CN(Q, li + 1, lj)));
+                        w + mpdata::pp(a) * mpdata::min3(T(1), CP(2, li + 1, lj), CN(2, li, lj)) -
+                            mpdata::pn(a) * mpdata::min3(T(1), CP(2, li, lj), CN(2, li + 1, lj)));
                 }
                 if (gj == mp) {
                     const T w = mpdata::donor(xc, bcy[(gi - 1) + np * ((Q - 1) + lp)], ldo(u2q + o.yx, so));
-                    const T a = sv2[Q & 1][pj + 1][pi];
+                    const T a = LMID(sv2, E0, 1, 0);
                     sto(pl(dio->m2, YXY, Q) + o.yx, so,
-                        w + mpdata::pp(a) * mpdata::min3(T(1), CP(Q, li, lj + 1), CN(Q, li, lj)) -
-                            mpdata::pn(a) * mpdata::min3(T(1), CP(Q, li, lj), CN(Q, li, lj + 1)));
+                        w + mpdata::pp(a) * mpdata::min3(T(1), CP(2, li, lj + 1), CN(2, li, lj)) -
+                            mpdata::pn(a) * mpdata::min3(T(1), CP(2, li, lj), CN(2, li, lj + 1)));
                 }
                 if (sky) {
                     const T w = -f3ns;
                     const T a = -v3q1;
                     sto(pl(dio->m3, SXY, Q + 1), so,
-                        w + mpdata::pp(a) * mpdata::min3(T(1), CP(Q - 1, li, lj), CN(Q, li, lj)) -
-                            mpdata::pn(a) * mpdata::min3(T(1), CP(Q, li, lj), CN(Q - 1, li, lj)));
+                        w + mpdata::pp(a) * mpdata::min3(T(1), CP(3, li, lj), CN(2, li, lj)) -
+                            mpdata::pn(a) * mpdata::min3(T(1), CP(2, li, lj), CN(3, li, lj)));
                 }
             }
         }
 #undef XA
 #undef CP
 #undef CN
-        // ---- rotate the computed queues to plane K+1 --------------------------
-        v3q1 = v3q2; v3q2 = v3n;
-        xmx0 = xmx1; xmn0 = xmn1;
+        // ---- the computed queues move to plane K+1 by the slot rotation; V3
+        //      of face K goes into the slot of face K-2 ------------------------
+        v3q1 = v3n;
+    };
+#undef LMID
+#undef LHI
+    for (int K = Ks;;) {
+        step(std::integral_constant<int, 0>{}, K);
+        if (++K > Ke) break;
+        step(std::integral_constant<int, 1>{}, K);
+        if (++K > Ke) break;
+        step(std::integral_constant<int, 2>{}, K);
+        if (++K > Ke) break;
+        step(std::integral_constant<int, 3>{}, K);
+        if (++K > Ke) break;
+        step(std::integral_constant<int, 4>{}, K);
+        if (++K > Ke) break;
+        step(std::integral_constant<int, 5>{}, K);
+        if (++K > Ke) break;
     }
 }
 
