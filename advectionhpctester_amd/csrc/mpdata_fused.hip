@@ -561,6 +561,14 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         asm volatile("" : "+v"(so_k), "+v"(fo_k));
         const uint32_t so = so_k, fo = fo_k;
         const bool A_on = K <= Alast, An_on = K + 1 <= Alast;
+        // h and rhr of the output plane K-2 for D (cache hits), loaded before
+        // this step's loads of plane K+1: a wait for them then does not wait
+        // for those too (loads complete in issue order for s_waitcnt).  h of
+        // plane K-1 for C is the queue's.
+        const T hQ = ldo(pl(h, SXY, kc(K - 2)), so);
+        T rQ = T(0);
+        if constexpr (!DENS) rQ = ldo(pl(rhr, SXY, kc(K - 2)), so);
+        const T hP = hq1;
         // x of plane K+1 for the LDS: ring cell, and the own cell where it
         // is not in the queue (positions outside the domain)
         T xr = T(0), xown = T(0);
@@ -742,13 +750,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             if constexpr (!DENS) rq2 = ldo(pl(rhr, SXY, kc(Kn)), so);
         }
         __syncthreads();
-        // h of plane P for C, h and rhr of the output plane Q = K-2 for D:
-        // loaded here (cache hits) rather than carried in registers
         const bool D_on = do_d && Dq;
-        const T hP = ldo(pl(h, SXY, kc(P)), so);
-        const T hQ = ldo(pl(h, SXY, kc(Q)), so);
-        T rQ = T(0);
-        if constexpr (!DENS) rQ = ldo(pl(rhr, SXY, kc(Q)), so);
         // ---- C: cp, cn on plane P ------------------------------------------
         if (BC_on) {
             const bool gnd = P == 1, sky = P == lp;
