@@ -733,10 +733,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                                 CP(2, li, lj - 1), CN(2, li, lj));
         }
         // The loaded source-column queues are not read after this point:
-        // rotate them to plane K+1 and load the new planes into the freed
-        // slots.  Each load lands in the register the next step reads, so
-        // no copy at the end of the step waits for it; its latency is
-        // covered by C, D and the next step's barrier.
+        // the oldest slot of each is loaded with the plane step K+1 needs;
+        // the latency is covered by C, D and the next step's barrier.
         {
             // the oldest slot of each queue becomes the newest of step K+1
             const int Kn = K + 1;
