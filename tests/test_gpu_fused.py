@@ -82,7 +82,9 @@ def test_fused_matches_stages_wide_halo(mp, ih, size):
 def test_fused_matches_stages_short_chunks(mp, cfg, chunk, monkeypatch):
     """k-chunks shorter than the two planes of xant and one of V/cp/cn each
     chunk recomputes below its first output plane (small domains take 4 or
-    8 planes): every chunk start, in both tilings, gives the stage bits."""
+    8 planes): every chunk start, in both tilings, gives the stage bits.
+    Over lp = 11 these chunk lengths give k-loop trip counts of every
+    residue mod 6, so the six-fold unrolled loop is left at each phase."""
     from advectionhpctester_amd.testdata import random_case
     monkeypatch.setenv("MPDATA_FUSED_CFG", cfg)
     monkeypatch.setenv("MPDATA_FUSED_CHUNK", chunk)
