@@ -1,0 +1,35 @@
+"""bench.py's command-line contract, checked without a GPU: more than one
+GPU is refused with a message, and without a GPU the benchmark fails
+instead of printing a number from some other path."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BENCH = os.path.join(ROOT, "bench.py")
+
+
+def _run(*args, timeout=600):
+    return subprocess.run([sys.executable, BENCH, *args], cwd=ROOT, capture_output=True,
+                          text=True, timeout=timeout)
+
+
+def test_more_than_one_gpu_is_refused():
+    r = _run("--gpus", "2")
+    assert r.returncode == 2
+    assert "one GPU" in r.stdout + r.stderr
+    assert "{" not in r.stdout
+
+
+def _gpu_visible():
+    import torch
+    return torch.cuda.device_count() > 0
+
+
+@pytest.mark.skipif(_gpu_visible(), reason="checks the behaviour on a host without a GPU")
+def test_no_gpu_fails_without_a_result():
+    r = _run("--steps", "1", "--warmup", "0", "--cpu-steps", "0")
+    assert r.returncode != 0
+    assert "{" not in r.stdout
