@@ -77,9 +77,10 @@ def test_fused_matches_stages_wide_halo(mp, ih, size):
     assert bits_equal(interior(a, ih), interior(b, ih)), describe_diff(interior(a, ih), interior(b, ih))
 
 
+@pytest.mark.parametrize("tdt,npdt", DTYPES)
 @pytest.mark.parametrize("cfg", ["0", "1"])
 @pytest.mark.parametrize("chunk", ["1", "2", "3", "4", "7"])
-def test_fused_matches_stages_short_chunks(mp, cfg, chunk, monkeypatch):
+def test_fused_matches_stages_short_chunks(mp, cfg, chunk, tdt, npdt, monkeypatch):
     """k-chunks shorter than the two planes of xant and one of V/cp/cn each
     chunk recomputes below its first output plane (small domains take 4 or
     8 planes): every chunk start, in both tilings, gives the stage bits.
@@ -89,15 +90,15 @@ def test_fused_matches_stages_short_chunks(mp, cfg, chunk, monkeypatch):
     monkeypatch.setenv("MPDATA_FUSED_CFG", cfg)
     monkeypatch.setenv("MPDATA_FUSED_CHUNK", chunk)
     dims = (61, 37, 11, 1)
-    c = random_case(*dims, seed=int(chunk) * 7 + int(cfg), dtype=np.float64)
-    with mp.Domain(*dims) as d:
+    c = random_case(*dims, seed=int(chunk) * 7 + int(cfg), dtype=npdt)
+    with mp.Domain(*dims, dtype=tdt) as d:
         t = {k: to_dev(v) for k, v in c.items()}
         xs = t["x"].clone()
         xo = torch.empty_like(t["x"])
         d.set_path(mp.PATH_STAGES)
         d.step(xs, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], t["bcx"], t["bcy"])
         d.set_path(mp.PATH_FUSED)
-        print(f"fused cfg={cfg} chunk={chunk}", flush=True)
+        print(f"fused cfg={cfg} chunk={chunk} {tdt}", flush=True)
         d.step_oop(t["x"], xo, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], t["bcx"], t["bcy"])
         a, b = to_host(xs), to_host(xo)
     assert bits_equal(interior(a), interior(b)), describe_diff(interior(a), interior(b))

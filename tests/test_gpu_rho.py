@@ -108,10 +108,12 @@ def test_rho_gauge_equals_oracle(mp, tdt, npdt, flags, cmax, liner, steps, path)
 
 
 @pytest.mark.parametrize("tdt,npdt", DTYPES)
-@pytest.mark.parametrize("cfg,chunk", [("0", "8"), ("1", "16"), ("1", "5")])
+@pytest.mark.parametrize("cfg,chunk", [("0", "8"), ("1", "16"), ("1", "5"), ("1", "3"), ("0", "6")])
 def test_rho_fused_tilings(mp, tdt, npdt, cfg, chunk, monkeypatch):
     """Both tilings of the fused kernel (32x32 and 32x16 lanes), several
-    k-chunks with a partial last one, tiles cut by both walls."""
+    k-chunks with a partial last one, tiles cut by both walls.  Over lp = 30
+    these chunk lengths give k-loop trip counts of every residue mod 6, so
+    the six-fold unrolled loop is left at each phase."""
     monkeypatch.setenv("MPDATA_FUSED_CFG", cfg)
     monkeypatch.setenv("MPDATA_FUSED_CHUNK", chunk)
     dims = (70, 45, 30, 1)
