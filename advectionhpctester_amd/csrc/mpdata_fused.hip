@@ -66,16 +66,17 @@ __device__ __forceinline__ void sto(T* p, uint32_t off, T v) {
     *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + off) = v;
 }
 
-// The value held by lane l-1 of the wave (DPP wave_shr:1; lane 0 gets 0).
+// The value held by lane l-1 of the wave (DPP wave_shr:1; lane 0, which has
+// no source lane, gets 0 through bound_ctrl, so no old value has to be set).
 // Convergent: called with every lane of the wave active.
 __device__ __forceinline__ double from_left_lane(double v) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, 0x138, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x138, 0xf, 0xf, false);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x138, 0xf, 0xf, true);
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 __device__ __forceinline__ float from_left_lane(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
