@@ -570,9 +570,12 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         T rQ = T(0);
         if constexpr (!DENS) rQ = ldo(pl(rhr, SXY, kc(K - 2)), so);
         const T hP = hq1;
+        // Temporaries of a step that are read only under the condition
+        // that sets them are left unset rather than zeroed: a default value
+        // would cost a register move per step on every path.
         // x of plane K+1 for the LDS: ring cell, and the own cell where it
         // is not in the queue (positions outside the domain)
-        T xr = T(0), xown = T(0);
+        T xr, xown;
         if (An_on) {
             if (t < NRING) xr = ld(ring, K + 1);
             if (!in_dom) xown = ld(own, K + 1);
@@ -598,7 +601,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         const T nv1_u3a = from_left_lane(u3q0);
         const T nv1_u3b = from_left_lane(u3q1);
         // column (gi, gj-1)
-        T nv2_h = T(0), nv2_u1a = T(0), nv2_u1b = T(0), nv2_u3a = T(0), nv2_u3b = T(0);
+        T nv2_h, nv2_u1a, nv2_u1b, nv2_u3a, nv2_u3b;  // read under BC_on && do_v2
         if constexpr (XJ<RX, RY>::value) {
             if (BC_on) {
                 LHI(e_h, 0) = hq1;  // element e = t
@@ -615,7 +618,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         }
 
         // ---- A: xant on plane K ------------------------------------------
-        T xa_new = T(0);
+        T xa_new;  // read under A_on, or in C where P < lp implies it
         if (A_on) {
             auto& sx = L.sx[E0];
             const bool gnd = K == 1, sky = K == lp;
@@ -677,7 +680,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         const int Q = K - 2;
         const bool Dq = Q >= k0 && Q <= k1;
         // ---- B: pair sums, V1/V2 on plane P=K-1, V3 on face K --------------
-        T s1n = T(0), s2n = T(0), v3n = T(0);
+        T s1n, s2n, v3n = T(0);  // s1n/s2n read in B's z terms, where A_on holds
         if (A_on) {
             if (do_v1) s1n = xa_new + XA(0, li - 1, lj);
             if (do_v2) s2n = xa_new + XA(0, li, lj - 1);
