@@ -619,9 +619,14 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
 
         // ---- A: xant on plane K ------------------------------------------
         T xa_new;  // read under A_on, or in C where P < lp implies it
-        if (A_on) {
+        // A and C are instantiated twice: for the ground and sky planes, and
+        // for the interior planes, where the boundary cases compile away
+        // (a uniform branch picks one; otherwise both sides of each
+        // boundary case would be computed and selected on every plane)
+        auto stage_a = [&](auto edge) __attribute__((always_inline)) {
+            constexpr bool EDGE = decltype(edge)::value;
             auto& sx = L.sx[E0];
-            const bool gnd = K == 1, sky = K == lp;
+            const bool gnd = EDGE && K == 1, sky = EDGE && K == lp;
             T hinv = T(1) / hq2;
             T xc = xq2;
             T f1p = donor(xc, sx[saj][sai + 1], u1b1);
@@ -654,6 +659,10 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             if (!sky) { m = max2(m, xq3); n = min2(n, xq3); }
             xmx1 = m;
             xmn1 = n;
+        };
+        if (A_on) {
+            if (K == 1 || K == lp) stage_a(std::true_type{});
+            else stage_a(std::false_type{});
         }
         if (An_on) {
             sx[E1][pj + 1][pi + 1] = in_dom ? xq3 : xown;
@@ -754,8 +763,9 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
         __syncthreads();
         const bool D_on = do_d && Dq;
         // ---- C: cp, cn on plane P ------------------------------------------
-        if (BC_on) {
-            const bool gnd = P == 1, sky = P == lp;
+        auto stage_c = [&](auto edge) __attribute__((always_inline)) {
+            constexpr bool EDGE = decltype(edge)::value;
+            const bool gnd = EDGE && P == 1, sky = EDGE && P == lp;
             T rp = T(0), rn = T(0);
             if (do_c) {
                 T mxo = xmx0, mno = xmn0;
@@ -781,6 +791,10 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             }
             LMID(scp, E1, 0, 0) = rp;
             LMID(scn, E1, 0, 0) = rn;
+        };
+        if (BC_on) {
+            if (P == 1 || P == lp) stage_c(std::true_type{});
+            else stage_c(std::false_type{});
         }
         // the lower z-face flux of the first output plane of the range is
         // the upper face of plane k0-1, formed here on the step before (the
