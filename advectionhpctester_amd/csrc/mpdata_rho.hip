@@ -219,7 +219,8 @@ __global__ void __launch_bounds__(256)
 
 // velprd_traj0's extrapolation (module_velprd.f90:55-85) followed, per cell,
 // by the time-level rotation (velprd_driver.f90:254-270): level n-1 of ox is
-// read before the rotation overwrites it with ox(.,0)*h.
+// read before the rotation overwrites it with ox(.,0)*h.  The twelve level
+// stores are non-temporal: nothing reads them back before they leave L2.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_extrap_rotate(Geo g, T* __restrict__ ox, T* __restrict__ oy, T* __restrict__ oz,
@@ -232,28 +233,28 @@ __global__ void __launch_bounds__(256)
     {
         const T a0 = ox[c] * hh, a2 = ox[c + 2 * n];
         const T a1 = a0 + r * (a0 - a2);
-        ox[c + n] = T(0.5) * (a1 + a0) * gc1;
-        ox[c + 2 * n] = ox[c] * hh;
+        __builtin_nontemporal_store(T(0.5) * (a1 + a0) * gc1, &ox[c + n]);
+        __builtin_nontemporal_store(ox[c] * hh, &ox[c + 2 * n]);
     }
     {
         const T a0 = oy[c] * hh, a2 = oy[c + 2 * n];
         const T a1 = a0 + r * (a0 - a2);
-        oy[c + n] = T(0.5) * (a1 + a0) * gc2;
-        oy[c + 2 * n] = oy[c] * hh;
+        __builtin_nontemporal_store(T(0.5) * (a1 + a0) * gc2, &oy[c + n]);
+        __builtin_nontemporal_store(oy[c] * hh, &oy[c + 2 * n]);
     }
     {
         const T a0 = oz[c] * hh, a2 = oz[c + 2 * n];
         const T a1 = a0 + r * (a0 - a2);
-        oz[c + n] = T(0.5) * (a1 + a0) * gc3;
-        oz[c + 2 * n] = oz[c] * hh;
+        __builtin_nontemporal_store(T(0.5) * (a1 + a0) * gc3, &oz[c + n]);
+        __builtin_nontemporal_store(oz[c] * hh, &oz[c + 2 * n]);
     }
     const T u0 = u[c], v0 = v[c], w0 = w[c];
-    u[c + n] = u0 + r * (u0 - u[c + 2 * n]);
-    v[c + n] = v0 + r * (v0 - v[c + 2 * n]);
-    w[c + n] = w0 + r * (w0 - w[c + 2 * n]);
-    u[c + 2 * n] = u0;
-    v[c + 2 * n] = v0;
-    w[c + 2 * n] = w0;
+    __builtin_nontemporal_store(u0 + r * (u0 - u[c + 2 * n]), &u[c + n]);
+    __builtin_nontemporal_store(v0 + r * (v0 - v[c + 2 * n]), &v[c + n]);
+    __builtin_nontemporal_store(w0 + r * (w0 - w[c + 2 * n]), &w[c + n]);
+    __builtin_nontemporal_store(u0, &u[c + 2 * n]);
+    __builtin_nontemporal_store(v0, &v[c + 2 * n]);
+    __builtin_nontemporal_store(w0, &w[c + 2 * n]);
 }
 
 dim3 cell_block() { return dim3(64, 4, 1); }
