@@ -22,8 +22,9 @@ Prints one JSON line:
              absent
 Sub-objects: unfused_stages (the per-stage kernels), ensemble (64 uneven
 tracers in one persistent launch / HIP graph), multifield (eight fields on one
-grid), density (the density step with the velocity predictor), adjoint
-(forward+backward against one forward).  --density-only prints the density
+grid), density (the density step with the velocity predictor), configs (the
+periodic-standard and deformational-flow configurations, per-stage kernels),
+adjoint (forward+backward against one forward).  --density-only prints the density
 object alone.
 """
 import argparse
@@ -64,6 +65,9 @@ def parse():
     ap.add_argument("--density-only", action="store_true",
                     help="run only the density-step measurement (--steps calls) and print its "
                          "JSON object")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the periodic-standard and deformational-flow configurations "
+                         "(per-stage kernels)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost measurement (128x128x64, 50 steps)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -192,6 +196,11 @@ def main():
         torch.cuda.empty_cache()
         dens = density_bench(20)
 
+    cfgs = None
+    if not a.no_configs and a.dtype == "f64":
+        torch.cuda.empty_cache()
+        cfgs = configs_bench(20)
+
     adj = None
     if not a.no_adjoint and a.dtype == "f64":
         torch.cuda.empty_cache()
@@ -227,6 +236,7 @@ def main():
         "ensemble": ens,
         "multifield": multi,
         "density": dens,
+        "configs": cfgs,
         "adjoint": adj,
         "checked_fused_equals_stages": checked,
         "wall_s_timed": wall,
@@ -361,6 +371,82 @@ def multifield_bench(steps, size=(256, 256, 128), nfields=8):
 # at n+1/2 and n-1, the mass fluxes u1,u2,u3, x_in, x_incomp, rhoadv, rhr,
 # pexr1, pexr2 (21).
 DENSITY_ARRAYS = 37
+
+
+def configs_bench(steps):
+    """The two configurations that run the per-stage kernels (periodic walls
+    and three iterations are outside the fused kernel):
+      periodic: 64x64x32, periodic in all directions, standard scheme, 2
+        iterations, limiter off, solid-body rotation about z at Courant 0.5,
+        Gaussian sphere (testdata.periodic_rotation_case); 100 steps;
+      deformational: 256x256x128, time-varying deformational flow at Courant
+        0.7 with the velocity predictor every step, gauge, 3 iterations,
+        limiter on, slotted cylinders, FP64 and FP32; a step is the
+        prediction to n+1/2, the MPDATA step and the level rotation.
+    Timed with events on torch's current stream after 2 warm-up steps."""
+    import numpy as np
+    import torch
+
+    from advectionhpctester_amd import mpdata, testdata
+    dev = torch.device("cuda:0")
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn, n):
+        fn(2)
+        torch.cuda.synchronize()
+        ev0.record()
+        fn(n)
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / n
+
+    out = {}
+    dims = (64, 64, 32)
+    c = testdata.periodic_rotation_case(*dims, seed=11)
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in c.items()}
+    with mpdata.Domain(*dims, ibc=(1, 1, 1)) as d:
+        def periodic(n):
+            for _ in range(n):
+                d.step(t["x"], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"],
+                       variant=mpdata.STANDARD, iters=2, nonos=0)
+        ms = timed(periodic, 100)
+    out["periodic_64x64x32_standard"] = {"ms_per_step": ms,
+                                         "value": dims[0] * dims[1] * dims[2] / (ms / 1e3)}
+
+    np_, mp_, lp_ = 256, 256, 128
+    period = 200
+    vel = testdata.deformational_velocity(np_, mp_, lp_, cmax=0.7)
+    x0 = testdata.slotted_cylinders(np_, mp_, lp_)
+    for tdt, name in ((torch.float64, "f64"), (torch.float32, "f32")):
+        U = [torch.from_numpy(np.ascontiguousarray(a)).to(dev, tdt) for a in vel]
+        x = torch.from_numpy(np.ascontiguousarray(x0)).to(dev, tdt)
+        h = torch.ones_like(x)
+        bcx = torch.zeros((2, lp_, mp_), dtype=tdt, device=dev)
+        bcy = torch.zeros((2, lp_, np_), dtype=tdt, device=dev)
+        lev = [torch.zeros((3,) + tuple(x.shape), dtype=tdt, device=dev) for _ in range(6)]
+        ox, oy, oz, uu, vv, ww = lev
+        with mpdata.Domain(np_, mp_, lp_, dtype=tdt) as d:
+            u1, u2, u3 = (d.zeros(k) for k in (1, 2, 3))
+            state = {"n": 0}
+
+            def deform(n):
+                for _ in range(n):
+                    f = testdata.reversal_factor(state["n"], period)
+                    for o, a in zip((ox, oy, oz), U):
+                        o[0] = a * f
+                        if state["n"] == 0:
+                            o[2] = o[0] * h
+                    d.velprd_traj0(ox, oy, oz, u1, u2, u3, h, 1.0, 1.0, 1.0, 1.0, 1.0)
+                    d.step(x, u1, u2, u3, h, h, bcx, bcy, variant=mpdata.GAUGE, iters=3, nonos=1)
+                    d.velprd_rotate(uu, vv, ww, ox, oy, oz, h, 1.0)
+                    state["n"] += 1
+            ms = timed(deform, steps)
+        out[f"deformational_256x256x128_gauge3_{name}"] = {
+            "ms_per_step": ms, "value": np_ * mp_ * lp_ / (ms / 1e3)}
+        del U, x, h, lev, ox, oy, oz, uu, vv, ww
+        torch.cuda.empty_cache()
+    out["steps"] = steps
+    return out
 
 
 def density_bench(steps, size=(256, 256, 128)):
