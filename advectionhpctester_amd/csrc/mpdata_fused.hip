@@ -14,10 +14,10 @@
 //   C  the limiter ratios cp,cn on plane K-1 (tile + 1-cell ring),
 //   D  the new x on plane K-2 (the tile),
 // keeping the last three xant planes and two V1/V2/cp/cn planes in LDS
-// rings and the V3 faces and the column's inputs in register queues.  Halo values follow the reference's boundary semantics directly
-// (inflow values for the upwind, mirrored xant, zero wall velocities,
-// edge-copied x for the limiter, zero cp/cn halo), so no halo-fill kernels
-// run.  Expressions are those of the per-stage kernels, operand for operand,
+// rings and the V3 faces and the column's inputs in register queues.  Halo
+// values follow the reference's boundary semantics directly (inflow values
+// for the upwind, mirrored xant, zero wall velocities, edge-copied x for
+// the limiter, zero cp/cn halo), so no halo-fill kernels run.  Expressions are those of the per-stage kernels, operand for operand,
 // so the fused result is bit-identical to them.
 #include "mpdata_fused.hpp"
 
