@@ -85,9 +85,11 @@ int pack_table_any(mpdata_ens* e, int cfg, int clen, int parity, std::vector<cha
 
 // Packs and uploads both parity tables for (cfg, clen).
 int build_tables(mpdata_ens* e, int cfg, int clen) {
+    // no step or graph replay may still read the old tables, and a graph
+    // exec is destroyed only once no replay of it runs
+    HIPCHK(hipDeviceSynchronize());
     drop_graph(e);
     e->tables_ok = false;
-    HIPCHK(hipDeviceSynchronize());  // no step may still read the old tables
     std::vector<char> host;
     int nitem = -1;
     for (int parity = 0; parity < 2; parity++) {
@@ -291,6 +293,8 @@ int mpdata_ens_run(mpdata_ens* e, int nsteps, int use_graph, void* stream) {
         return MPDATA_OK;
     }
     if (!e->exec || e->graph_steps != nsteps) {
+        // the previous graph may still be replaying on any stream
+        if (e->exec) HIPCHK(hipDeviceSynchronize());
         drop_graph(e);
         if (!e->cap) HIPCHK(hipStreamCreateWithFlags(&e->cap, hipStreamNonBlocking));
         HIPCHK(hipStreamBeginCapture(e->cap, hipStreamCaptureModeRelaxed));

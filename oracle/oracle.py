@@ -4,9 +4,11 @@ Two libraries are wrapped:
 
 * ``liboracle_mpdata.so``: the C restatement of the reference MPDATA path
   (``oracle/mpdata_oracle_impl.inc``), built by ``oracle/Makefile``.
-* ``_ref/libmpdata_ref.so``: the reference's own Fortran routines compiled
-  from the advectionHPCtester sources by ``oracle/build_ref.sh`` with the
-  bind(C) shim ``oracle/ref_shim.f90``.  Present only where it was built.
+* ``_ref/libmpdata_ref.so`` and ``_ref/libmpdata_ref32.so``: the reference's
+  own Fortran routines compiled from the advectionHPCtester sources by
+  ``oracle/build_ref.sh`` with the bind(C) shim ``oracle/ref_shim.f90``, in
+  its default FP64 kind and with its -DFLOAT_PRECISION=4 switch (FP32).
+  Present only where they were built.
 
 Only tests/, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg
 import this module, and only as the checker.
@@ -25,6 +27,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle_mpdata.so")
 REF_SO = os.path.join(HERE, "_ref", "libmpdata_ref.so")
+REF32_SO = os.path.join(HERE, "_ref", "libmpdata_ref32.so")
 
 UPWIND, GAUGE, STANDARD = 0, 1, 2
 
@@ -163,55 +166,69 @@ class Oracle:
 
 
 class Reference:
-    """The reference's own routines (FP64 build), through oracle/ref_shim.f90.
+    """The reference's own routines through oracle/ref_shim.f90: the FP64 build
+    (``dtype`` float64) or the -DFLOAT_PRECISION=4 build (float32).  Scalar
+    arguments cross the shim as C doubles and are converted to the build's
+    real kind inside it, as the reference converts its parameters.
 
     The reference keeps its domain and scratch in module globals, so a
     ``setup`` call fixes the grid for all following calls."""
 
-    def __init__(self, path=REF_SO):
+    def __init__(self, dtype=np.float64, path=None):
+        self.dtype = np.dtype(dtype)
+        path = path or (REF_SO if self.dtype == np.float64 else REF32_SO)
         if not os.path.exists(path):
             raise FileNotFoundError(f"{path} not built (oracle/build_ref.sh)")
-        self.lib = ctypes.CDLL(path)
+        # a separate handle per build: both export the same symbol names
+        self.lib = ctypes.CDLL(path, mode=os.RTLD_LOCAL)
+        kind = self.lib.ref_precision()
+        if kind != self.dtype.itemsize:
+            raise RuntimeError(f"{path}: real kind {kind}, expected {self.dtype.itemsize}")
         self.dims = None
 
+    def _p(self, a):
+        if a is not None:
+            assert a.dtype == self.dtype, (a.dtype, self.dtype)
+        return _ptr(a)
+
     @staticmethod
-    def available(path=REF_SO):
-        return os.path.exists(path)
+    def available(dtype=np.float64):
+        return os.path.exists(REF_SO if np.dtype(dtype) == np.float64 else REF32_SO)
 
     def setup(self, np_, mp, lp, ih=1, ibc=(0, 0, 0), dx00=100.0, nt=556, dt=0.0):
         self.lib.ref_setup(np_, mp, lp, ih, *ibc, ctypes.c_double(dx00), nt, ctypes.c_double(dt))
         self.dims = (np_, mp, lp, ih)
 
     def upwind(self, u1, u2, u3, x, xant, rhr, h, bcx, bcy):
-        self.lib.ref_upwind(*map(_ptr, (u1, u2, u3, x, xant, rhr, h, bcx, bcy)))
+        self.lib.ref_upwind(*map(self._p, (u1, u2, u3, x, xant, rhr, h, bcx, bcy)))
 
     def antidiff(self, variant, u1, u2, u3, xant, x, rhr, h, out=None):
         o = out or {}
         names = ("v1", "v2", "v3", "cp", "cn", "f1", "f2", "f3", "f1ns", "f2ns", "f3ns")
-        self.lib.ref_antidiff(variant, *map(_ptr, (u1, u2, u3, xant, x, rhr, h)),
-                              *[_ptr(o.get(n)) for n in names])
+        self.lib.ref_antidiff(variant, *map(self._p, (u1, u2, u3, xant, x, rhr, h)),
+                              *[self._p(o.get(n)) for n in names])
 
     def advec_dwarf(self, legacyoptmode, t_adv, x, bcx, bcy, u1, u2, u3, rhr, h):
-        self.lib.ref_advec_dwarf(legacyoptmode, t_adv, *map(_ptr, (x, bcx, bcy, u1, u2, u3, rhr, h)))
+        self.lib.ref_advec_dwarf(legacyoptmode, t_adv, *map(self._p, (x, bcx, bcy, u1, u2, u3, rhr, h)))
 
     def set_initial15(self, u1, u2, u3, x, rho):
-        self.lib.ref_set_initial15(*map(_ptr, (u1, u2, u3, x, rho)))
+        self.lib.ref_set_initial15(*map(self._p, (u1, u2, u3, x, rho)))
 
     def norms(self, x, x0, icnt):
-        self.lib.ref_norms(_ptr(x), _ptr(x0), icnt)
+        self.lib.ref_norms(self._p(x), self._p(x0), icnt)
 
     def velprdA_to_C(self, ver, ox, oy, oz, u1, u2, u3):
-        self.lib.ref_velprdA_to_C(ver, *map(_ptr, (ox, oy, oz, u1, u2, u3)))
+        self.lib.ref_velprdA_to_C(ver, *map(self._p, (ox, oy, oz, u1, u2, u3)))
 
     def velprd_traj0(self, ox, oy, oz, u1, u2, u3, h, dxi, dyi, dzi, dt, dt_ratio):
         D = ctypes.c_double
-        self.lib.ref_velprd_traj0(*map(_ptr, (ox, oy, oz, u1, u2, u3, h)), D(dxi), D(dyi), D(dzi),
+        self.lib.ref_velprd_traj0(*map(self._p, (ox, oy, oz, u1, u2, u3, h)), D(dxi), D(dyi), D(dzi),
                                   D(dt), D(dt_ratio))
 
     def compute_courants(self, ox, oy, oz, h, ntimelevel, dxi, dyi, dzi, dt):
         D = ctypes.c_double
         out = (D * 7)()
-        self.lib.ref_compute_courants(*map(_ptr, (ox, oy, oz, h)), ntimelevel, D(dxi), D(dyi),
+        self.lib.ref_compute_courants(*map(self._p, (ox, oy, oz, h)), ntimelevel, D(dxi), D(dyi),
                                       D(dzi), D(dt), out)
         return np.array(list(out))
 
@@ -229,6 +246,6 @@ class Reference:
         names = ("ox", "oy", "oz", "u", "v", "w", "u1", "u2", "u3", "x_in", "x_incomp", "h",
                  "rhoadv", "rhr", "pexe", "pexr1", "pexr2", "bcx", "bcy")
         self.lib.ref_rho_gauge(int(bool(only_driver)), fl, isub, rdt,
-                               *[_ptr(f[k]) for k in names], sp, ctypes.byref(lin))
+                               *[self._p(f[k]) for k in names], sp, ctypes.byref(lin))
         return {"nsubsteps": isub[0], "nsubsteps_o": isub[1], "dtn": rdt[0], "dtnold": rdt[1],
                 "status": 0}

@@ -31,3 +31,12 @@ def ref():
     if not Reference.available():
         pytest.skip("oracle/_ref/libmpdata_ref.so not built (needs the reference tree)")
     return Reference()
+
+
+@pytest.fixture(scope="session")
+def ref32():
+    import numpy as np
+    from oracle import Reference
+    if not Reference.available(np.float32):
+        pytest.skip("oracle/_ref/libmpdata_ref32.so not built (needs the reference tree)")
+    return Reference(np.float32)

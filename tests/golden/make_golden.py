@@ -1,8 +1,11 @@
 """Writes the golden fixtures of tests/golden from the reference's own compiled
-Fortran (oracle/_ref/libmpdata_ref.so, built by oracle/build_ref.sh from the
-advectionHPCtester sources).  Run in the container that holds the reference:
+Fortran (oracle/_ref/libmpdata_ref.so and, for the FP32 fixtures, the
+reference's -DFLOAT_PRECISION=4 build oracle/_ref/libmpdata_ref32.so, both
+built by oracle/build_ref.sh from the advectionHPCtester sources).  Run in
+the container that holds the reference:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py          # FP64 fixtures
+    python tests/golden/make_golden.py --fp32   # FP32 fixtures (*32*)
 
 Each fixture is data only: seeded inputs and the reference's outputs.
 """
@@ -30,15 +33,16 @@ def interior_hash(x, ih=1):
 
 def stage_case(ref, n, lp, seed):
     dims = (n, n, lp, 1)
-    c = testdata.random_case(*dims, seed=seed)
+    dt = ref.dtype
+    c = testdata.random_case(*dims, seed=seed, dtype=dt)
     ref.setup(n, n, lp, 1)
     out = {f"in_{k}": v for k, v in c.items()}
-    x, xant = c["x"].copy(), np.zeros(shape(*dims))
+    x, xant = c["x"].copy(), np.zeros(shape(*dims), dt)
     ref.upwind(c["u1"], c["u2"], c["u3"], x, xant, c["rhr"], c["h"], c["bcx"], c["bcy"])
     out["up_x"], out["up_xant"] = x.copy(), xant.copy()
     for variant, name in ((1, "gauge"), (2, "standard")):
         xv, av = x.copy(), xant.copy()
-        s = {k: np.zeros(shape(*dims, kind)) for k, kind in
+        s = {k: np.zeros(shape(*dims, kind), dt) for k, kind in
              (("v1", 1), ("v2", 2), ("v3", 3), ("cp", 0), ("cn", 0), ("f1", 1), ("f2", 2),
               ("f3", 3), ("f1ns", 1), ("f2ns", 2), ("f3ns", 3))}
         ref.antidiff(variant, c["u1"], c["u2"], c["u3"], av, xv, c["rhr"], c["h"], s)
@@ -53,21 +57,22 @@ def stage_case(ref, n, lp, seed):
 def velprd_case(ref, n, lp, seed):
     rng = np.random.default_rng(seed)
     dims = (n, n, lp, 1)
+    dt = ref.dtype
     ref.setup(n, n, lp, 1)
     out = {}
     S = shape(*dims)
-    ox, oy, oz = (rng.uniform(-1, 1, (3,) + S) for _ in range(3))
-    h = rng.uniform(0.8, 1.2, S)
+    ox, oy, oz = (rng.uniform(-1, 1, (3,) + S).astype(dt) for _ in range(3))
+    h = rng.uniform(0.8, 1.2, S).astype(dt)
     out["in_ox"], out["in_oy"], out["in_oz"], out["in_h"] = ox.copy(), oy.copy(), oz.copy(), h
-    u1, u2, u3 = (np.zeros(shape(*dims, k)) for k in (1, 2, 3))
+    u1, u2, u3 = (np.zeros(shape(*dims, k), dt) for k in (1, 2, 3))
     ref.velprd_traj0(ox, oy, oz, u1, u2, u3, h, 0.7, 0.9, 1.1, 0.05, 0.8)
     out["traj0_ox"], out["traj0_oy"], out["traj0_oz"] = ox, oy, oz
     out["traj0_u1"], out["traj0_u2"], out["traj0_u3"] = u1, u2, u3
-    a = [rng.uniform(-1, 1, S) for _ in range(3)]
+    a = [rng.uniform(-1, 1, S).astype(dt) for _ in range(3)]
     out["a2c_in"] = np.stack(a)
     for ver in (1, 2):
         aa = [v.copy() for v in a]
-        c1, c2, c3 = (np.zeros(shape(*dims, k)) for k in (1, 2, 3))
+        c1, c2, c3 = (np.zeros(shape(*dims, k), dt) for k in (1, 2, 3))
         ref.velprdA_to_C(ver, *aa, c1, c2, c3)
         out[f"a2c{ver}_u1"], out[f"a2c{ver}_u2"], out[f"a2c{ver}_u3"] = c1, c2, c3
         out[f"a2c{ver}_a"] = np.stack(aa)
@@ -143,6 +148,34 @@ def rho_cases(ref):
     return json.dumps(out, indent=1, sort_keys=True)
 
 
+def dwarf32(ref, steps_hash=(1, 50, 100)):
+    """The 59^3 dwarf in the reference's FP32 build: its own set_initial15
+    velocities (single-precision arithmetic, so they differ from an FP64
+    computation cast to FP32 by up to one unit in the last place), and
+    interior hashes of its trajectory for both antidiffusive variants."""
+    case = testdata.dwarf_setup(dtype=np.float32)
+    n = case.n
+    ref.setup(n, n, n, 1, dt=case.dt, nt=case.nt)
+    u1, u2, u3 = (np.zeros(shape(n, n, n, 1, k), np.float32) for k in (1, 2, 3))
+    x0 = np.zeros(shape(n, n, n, 1), np.float32)
+    ref.set_initial15(u1, u2, u3, x0, np.ones(shape(n, n, n, 1), np.float32))
+    out = {"u1": u1, "u2": u2, "u3": u3, "x0_sha256": np.array(interior_hash(x0))}
+    for variant, name in ((1, "gauge"), (2, "standard")):
+        x = x0.copy()
+        for it in range(1, max(steps_hash) + 1):
+            ref.advec_dwarf(4, variant, x, case.bcx, case.bcy, u1, u2, u3, case.rhr, case.h)
+            if it in steps_hash:
+                out[f"{name}_step{it}_sha256"] = np.array(interior_hash(x))
+    return out
+
+
+def main32():
+    ref = Reference(np.float32)
+    np.savez_compressed(os.path.join(HERE, "stage32_n9_l6.npz"), **stage_case(ref, 9, 6, 2))
+    np.savez_compressed(os.path.join(HERE, "velprd32_n8_l6.npz"), **velprd_case(ref, 8, 6, 3))
+    np.savez_compressed(os.path.join(HERE, "dwarf59_reference32.npz"), **dwarf32(ref))
+
+
 def main():
     ref = Reference()
     with open(os.path.join(HERE, "rho_reference.json"), "w") as fh:
@@ -160,4 +193,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main32() if "--fp32" in sys.argv[1:] else main()
