@@ -45,6 +45,7 @@ def _proto(real):
         "mpdata_velprdA_to_C": [_P] * 8,
         "mpdata_velprd_rotate": [_P] * 8 + [R, _P],
         "mpdata_compute_courants": [_P] * 5 + [_I] + [R] * 4 + [_P, _P],
+        "mpdata_compute_courants_noalloc": [_P] * 4 + [_I] + [R] * 4 + [_I] * 4 + [_P, _P],
         "mpdata_norm_sums": [_P] * 5,
         "mpdata_ce_advection": [_P, _I, _I, _I] + [_P] * 4 + [_P] * 5 + [_P],
         "mpdata_ce_advection_oop": [_P, _I, _I, _I] + [_P] * 5 + [_P] * 5 + [_P],

@@ -313,6 +313,24 @@ int mpdata_compute_courants_f64(mpdata_ctx *ctx, const double *ox, const double 
 int mpdata_compute_courants_f32(mpdata_ctx *ctx, const float *ox, const float *oy,
                                 const float *oz, const float *h, int ntimelevel, float dxi,
                                 float dyi, float dzi, float dt, double *out, void *stream);
+/* No counterpart in the reference.  The Courant maxima of
+ * mpdata_compute_courants_* for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight time loops, harnesses that forbid
+ * allocation): no context, no hipMalloc/hipFree, asynchronous on `stream`.
+ * The reduction partials live in a static device array of the library
+ * (3.67 MB per GPU, held until the process ends), which bounds the domain:
+ * ceil(np/64)*ceil(mp/4)*lp <= 65536 (e.g. 256x256x256); larger domains are
+ * rejected with MPDATA_EINVAL.  out: DEVICE memory, 7 doubles (cr3d, crxy,
+ * crxz, cryz, crx, cry, crz), the same values as mpdata_compute_courants_*.
+ * Calls that overlap on one GPU share the array and race. */
+int mpdata_compute_courants_noalloc_f64(const double *ox, const double *oy, const double *oz,
+                                        const double *h, int ntimelevel, double dxi,
+                                        double dyi, double dzi, double dt, int np, int mp,
+                                        int lp, int ih, double *out, void *stream);
+int mpdata_compute_courants_noalloc_f32(const float *ox, const float *oy, const float *oz,
+                                        const float *h, int ntimelevel, float dxi, float dyi,
+                                        float dzi, float dt, int np, int mp, int lp, int ih,
+                                        double *out, void *stream);
 /* Interior sums for the dwarf's norms (advection_test_setup.f90:472-492,
  * 595-617): out[4] (host) = sum x^2, sum x0^2, sum (x0-x)^2, sum x
  * (accumulated in FP64).  x0 may be NULL (its terms are then 0). */
