@@ -16,7 +16,7 @@ LIB_PATH = os.environ.get("MPDATA_HIP_LIB") or os.path.join(HERE, "libmpdata_hip
 MPDATA_OK, MPDATA_EINVAL, MPDATA_EHIP, MPDATA_ENOMEM, MPDATA_EUNSUPPORTED = 0, 1, 2, 3, 4
 MPDATA_ESTOP = 5
 UPWIND, GAUGE, STANDARD = 0, 1, 2
-PATH_AUTO, PATH_STAGES, PATH_FUSED = 0, 1, 2
+PATH_AUTO, PATH_STAGES, PATH_FUSED, PATH_FAST = 0, 1, 2, 3
 
 _P, _I, _D, _F, _LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_float, ctypes.c_longlong
 
@@ -43,6 +43,7 @@ def _proto(real):
         "mpdata_c_timeloop": [_P, _I, _I, _I, _I] + [_P] * 9,
         "mpdata_velprd_traj0": [_P] * 8 + [R] * 5 + [_P],
         "mpdata_velprdA_to_C": [_P] * 8,
+        "mpdata_velprd_c": [_P] * 8 + [R] * 5 + [_P] * 4,
         "mpdata_velprd_rotate": [_P] * 8 + [R, _P],
         "mpdata_compute_courants": [_P] * 5 + [_I] + [R] * 4 + [_P, _P],
         "mpdata_compute_courants_noalloc": [_P] * 4 + [_I] + [R] * 4 + [_I] * 4 + [_P, _P],

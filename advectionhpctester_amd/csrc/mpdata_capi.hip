@@ -244,11 +244,38 @@ int check_step_opts(int variant, int iters, int nonos) {
 template <typename T>
 int use_fused(mpdata_ctx* c, int variant, int iters, int nonos, bool* fused) {
     const bool fusable = Fused<T>::supports(c->g, variant, iters, nonos);
-    if (c->path == MPDATA_PATH_FUSED && !fusable)
+    if ((c->path == MPDATA_PATH_FUSED || c->path == MPDATA_PATH_FAST) && !fusable)
         return fail(MPDATA_EUNSUPPORTED,
                     "fused path supports variant %d iters %d nonos %d only for %s", variant,
                     iters, nonos, Fused<T>::scope());
     *fused = c->path != MPDATA_PATH_STAGES && fusable;
+    return MPDATA_OK;
+}
+
+// One fused step out of place: the fast build under MPDATA_PATH_FAST, else
+// the exactly rounded one.  Three iterations keep the second iterate and the
+// second pass's limited velocities in the context's scratch.
+template <typename T>
+int fused_step(mpdata_ctx* c, int iters, const T* x, T* xout, const T* u1, const T* u2,
+               const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+               hipStream_t s) {
+    FusedScratch<T> sc;
+    if (iters == 3) {
+        Scratch& S = scr<T>(c);
+        int rc;
+        if ((rc = ensure(c, &S.psi2, SCAL, sizeof(T))) ||
+            (rc = ensure(c, &S.ul[0], FACEX, sizeof(T))) ||
+            (rc = ensure(c, &S.ul[1], FACEY, sizeof(T))) ||
+            (rc = ensure(c, &S.ul[2], FACEZ, sizeof(T))))
+            return rc;
+        sc.x2 = (T*)S.psi2;
+        for (int d = 0; d < 3; d++) sc.ul[d] = (T*)S.ul[d];
+    }
+    const hipError_t e =
+        c->path == MPDATA_PATH_FAST
+            ? FusedFast<T>::step(c->g, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s, iters, &sc)
+            : Fused<T>::step(c->g, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s, iters, &sc);
+    if (e != hipSuccess) return hip_fail(e, "fused step");
     return MPDATA_OK;
 }
 
@@ -277,7 +304,7 @@ int step_impl(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T* u
         return step_stages<T>(c, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy, s);
     if ((rc = ensure(c, &S.xnext, SCAL, sizeof(T)))) return rc;
     T* xn = (T*)S.xnext;
-    HIPCHK(Fused<T>::step(c->g, x, xn, u1, u2, u3, rhr, h, bcx, bcy, s));
+    if ((rc = fused_step<T>(c, iters, x, xn, u1, u2, u3, rhr, h, bcx, bcy, s))) return rc;
     HIPCHK(hipMemcpyAsync(x, xn, (size_t)len_of<SCAL>(c->g) * sizeof(T), hipMemcpyDeviceToDevice, s));
     return MPDATA_OK;
 }
@@ -297,8 +324,7 @@ int step_oop_impl(mpdata_ctx* c, int variant, int iters, int nonos, const T* x_i
     if (a < b + bytes && b < a + bytes) return fail(MPDATA_EINVAL, "x_in and x_out overlap");
     if ((rc = use_fused<T>(c, variant, iters, nonos, &fused))) return rc;
     if (fused) {
-        HIPCHK(Fused<T>::step(c->g, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, s));
-        return MPDATA_OK;
+        return fused_step<T>(c, iters, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, s);
     }
     HIPCHK(hipMemcpyAsync(x_out, x_in, bytes, hipMemcpyDeviceToDevice, s));
     return step_stages<T>(c, variant, iters, nonos, x_out, u1, u2, u3, rhr, h, bcx, bcy, s);
@@ -326,7 +352,9 @@ int timeloop_impl(mpdata_ctx* c, int variant, int iters, int nonos, int nt, T* x
     if ((rc = ensure(c, &S.xnext, SCAL, sizeof(T)))) return rc;
     T* buf[2] = {x, (T*)S.xnext};
     for (int it = 0; it < nt; it++)
-        HIPCHK(Fused<T>::step(c->g, buf[it & 1], buf[(it + 1) & 1], u1, u2, u3, rhr, h, bcx, bcy, s));
+        if ((rc = fused_step<T>(c, iters, buf[it & 1], buf[(it + 1) & 1], u1, u2, u3, rhr, h, bcx,
+                                bcy, s)))
+            return rc;
     if (nt & 1)
         HIPCHK(hipMemcpyAsync(x, buf[1], (size_t)len_of<SCAL>(c->g) * sizeof(T),
                               hipMemcpyDeviceToDevice, s));
@@ -425,7 +453,7 @@ int mpdata_ctx_destroy(mpdata_ctx* ctx) {
 
 int mpdata_ctx_set_path(mpdata_ctx* ctx, int path) {
     if (!ctx) return fail(MPDATA_EINVAL, "null context");
-    if (path < MPDATA_PATH_AUTO || path > MPDATA_PATH_FUSED) return fail(MPDATA_EINVAL, "path %d", path);
+    if (path < MPDATA_PATH_AUTO || path > MPDATA_PATH_FAST) return fail(MPDATA_EINVAL, "path %d", path);
     ctx->path = path;
     return MPDATA_OK;
 }
@@ -533,6 +561,20 @@ long long mpdata_ctx_scratch_bytes(const mpdata_ctx* ctx) { return ctx ? ctx->he
         HIPCHK(Stages<T>::velprd_extrap(g, ox, oy, oz, h, dt * dxi, dt * dyi, dt * dzi, dt_ratio, \
                                         s));                                                     \
         HIPCHK(Stages<T>::a_to_c(g, ox + n, oy + n, oz + n, u1, u2, u3, s));                     \
+        return MPDATA_OK;                                                                        \
+    }                                                                                            \
+    int mpdata_velprd_c##SFX(mpdata_ctx* ctx, const T* ox, const T* oy, const T* oz,             \
+                             const T* oxp, const T* oyp, const T* ozp, const T* h, T dxi, T dyi, \
+                             T dzi, T dt, T dt_ratio, T* u1, T* u2, T* u3, void* stream) {      \
+        int rc;                                                                                  \
+        if ((rc = check_ctx(ctx, RB))) return rc;                                                \
+        if (any_null(ox, oy, oz, oxp, oyp, ozp, h, u1, u2, u3))                                  \
+            return fail(MPDATA_EINVAL, "null field");                                            \
+        const Geo& g = ctx->g;                                                                   \
+        if (g.px || g.py || g.pz)                                                                \
+            return fail(MPDATA_EUNSUPPORTED, "velprd_c: open boundaries only");                 \
+        HIPCHK(Stages<T>::predict_ac(g, ox, oy, oz, oxp, oyp, ozp, h, dt * dxi, dt * dyi,        \
+                                     dt * dzi, dt_ratio, u1, u2, u3, (hipStream_t)stream));      \
         return MPDATA_OK;                                                                        \
     }                                                                                            \
     int mpdata_velprdA_to_C##SFX(mpdata_ctx* ctx, T* ox, T* oy, T* oz, T* u1, T* u2, T* u3,      \

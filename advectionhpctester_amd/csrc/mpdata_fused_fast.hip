@@ -1,0 +1,38 @@
+// The fused gauge step of the fast build: the kernel templates of
+// mpdata_fused_impl.hpp with FAST = true (Newton-refined reciprocal
+// division), and this file compiled with -ffp-contract=fast so that products
+// and sums contract into FMAs.  Not bit-identical to the reference's
+// operation order; the exact build is mpdata_fused.hip.
+#include "mpdata_fused_impl.hpp"
+
+namespace mpdata {
+
+template <typename T>
+hipError_t FusedFast<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                              const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+                              hipStream_t s, int iters, const FusedScratch<T>* sc) {
+    const Off o = make_off(g);
+    int cfg = 0, cl = 64;
+    choose_tiling<T>(g, cfg, cl);
+    if (iters == 3) {
+        if (!sc || !sc->x2 || !sc->ul[0] || !sc->ul[1] || !sc->ul[2]) return hipErrorInvalidValue;
+        const hipError_t e =
+            launch_three<T, true>(g, o, cfg, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, *sc, s);
+        if (e != hipSuccess) return e;
+        k_copy_untouched_halo<T><<<256, 256, 0, s>>>(g, x, xout);
+        return hipGetLastError();
+    }
+    // the two tilings the chooser picks (32x32 lanes, 32x16 lanes at four
+    // waves per SIMD); the tuning overrides of the exact build are not built
+    const hipError_t e =
+        cfg == 0 ? launch2<T, 32, 32, 4, true>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s)
+                 : launch2<T, 32, 16, 4, true>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s);
+    if (e != hipSuccess) return e;
+    k_copy_untouched_halo<T><<<256, 256, 0, s>>>(g, x, xout);
+    return hipGetLastError();
+}
+
+template struct FusedFast<double>;
+template struct FusedFast<float>;
+
+}  // namespace mpdata

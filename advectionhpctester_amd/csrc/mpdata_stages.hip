@@ -524,6 +524,68 @@ __global__ void k_a_to_c(Geo g, const T* __restrict__ ox, const T* __restrict__ 
         Z_(u3, i, j, k) = T(0.5) * (S_(oz, i, j, k) + S_(oz, i, j, k - 1));
 }
 
+// The predictor in one pass (open walls): velprd_traj0's extrapolation of the
+// A-grid velocity to n+1/2 (module_velprd.f90:15-89), the one-sided
+// extrapolation into the wall halo and the averaging to the C grid
+// (velprdA_to_C, :166-364), without storing the A-grid level n+1/2.  Level
+// n-1 is read as the previous step's level-n array oxp times h, the value the
+// reference's level rotation stores in ox(.,2) (velprd_driver.f90:254-270),
+// so the caller rotates the time levels by swapping pointers; the products,
+// sums and their order are those of k_velprd_extrap, k_extrap_halo and
+// k_a_to_c, so u1..u3 are bit-identical to them as long as h is the same.
+// One thread per (i, j, k) of the face ranges; the level-n+1/2 values of the
+// neighbouring cells are recomputed from their inputs (cache hits).
+template <typename T>
+__device__ __forceinline__ T half_level(const T* __restrict__ a, const T* __restrict__ ap,
+                                        const T* __restrict__ h, long long c, T gc, T r) {
+    const T hh = h[c];
+    const T a0 = a[c] * hh, a2 = ap[c] * hh;
+    const T a1 = a0 + r * (a0 - a2);
+    return T(0.5) * (a1 + a0) * gc;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_predict_ac(Geo g, const T* __restrict__ ox, const T* __restrict__ oy, const T* __restrict__ oz,
+             const T* __restrict__ oxp, const T* __restrict__ oyp, const T* __restrict__ ozp,
+             const T* __restrict__ h, T gc1, T gc2, T gc3, T r, T* __restrict__ u1,
+             T* __restrict__ u2, T* __restrict__ u3) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = 1 + blockIdx.z;
+    const int np = g.np, mp = g.mp, lp = g.lp;
+    auto cell = [&](int ii, int jj, int kk) { return off_raw<SCAL>(g, ii, jj, kk); };
+    // level n+1/2 at cells m (upstream) and c (downstream) of a face at
+    // position q along a direction of n cells: the wall faces use the
+    // extrapolated halo cell 2a(1)-a(2) / 2a(n)-a(n-1)
+    auto face = [&](const T* a, const T* ap, T gc, int q, int n, auto at) -> T {
+        T lm, lc;
+        if (q == 1) {
+            const T l1 = half_level(a, ap, h, at(1), gc, r), l2 = half_level(a, ap, h, at(2), gc, r);
+            lm = T(2) * l1 - l2;
+            lc = l1;
+        } else if (q == n + 1) {
+            const T ln = half_level(a, ap, h, at(n), gc, r);
+            const T lnm = half_level(a, ap, h, at(n - 1), gc, r);
+            lm = ln;
+            lc = T(2) * ln - lnm;
+        } else {
+            lm = half_level(a, ap, h, at(q - 1), gc, r);
+            lc = half_level(a, ap, h, at(q), gc, r);
+        }
+        return T(0.5) * (lc + lm);
+    };
+    if (i <= np + 1 && j <= mp && k <= lp)
+        u1[off_raw<FACEX>(g, i, j, k)] =
+            face(ox, oxp, gc1, i, np, [&](int q) { return cell(q, j, k); });
+    if (i <= np && j <= mp + 1 && k <= lp)
+        u2[off_raw<FACEY>(g, i, j, k)] =
+            face(oy, oyp, gc2, j, mp, [&](int q) { return cell(i, q, k); });
+    if (i <= np && j <= mp && k <= lp + 1)
+        u3[off_raw<FACEZ>(g, i, j, k)] =
+            face(oz, ozp, gc3, k, lp, [&](int q) { return cell(i, j, q); });
+}
+
 // Time-level rotation: velprd_driver.f90:254-270
 template <typename T>
 __global__ void k_velprd_rotate(Geo g, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
@@ -785,6 +847,15 @@ hipError_t Stages<T>::a_to_c(const Geo& g, T* ox, T* oy, T* oz, T* u1, T* u2, T*
         k_extrap_halo<T><<<grid, wall_block(), 0, s>>>(g, oz, 2);
     }
     k_a_to_c<T><<<cell_grid(g, 1), cell_block(), 0, s>>>(g, ox, oy, oz, u1, u2, u3);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t Stages<T>::predict_ac(const Geo& g, const T* ox, const T* oy, const T* oz,
+                                 const T* oxp, const T* oyp, const T* ozp, const T* h, T gc1,
+                                 T gc2, T gc3, T dt_ratio, T* u1, T* u2, T* u3, hipStream_t s) {
+    k_predict_ac<T><<<cell_grid(g, 1), cell_block(), 0, s>>>(g, ox, oy, oz, oxp, oyp, ozp, h, gc1,
+                                                             gc2, gc3, dt_ratio, u1, u2, u3);
     return hipGetLastError();
 }
 

@@ -33,6 +33,12 @@ struct Stages {
                                     T gc3, T dt_ratio, hipStream_t s);
     static hipError_t a_to_c(const Geo& g, T* ox, T* oy, T* oz, T* u1, T* u2, T* u3,
                              hipStream_t s);
+    // velprd_extrap + a_to_c in one pass without storing the A-grid level
+    // n+1/2, level n-1 read as oxp*h (open walls; the caller rotates the time
+    // levels by swapping the level-n and level-(n-1) arrays)
+    static hipError_t predict_ac(const Geo& g, const T* ox, const T* oy, const T* oz,
+                                 const T* oxp, const T* oyp, const T* ozp, const T* h, T gc1,
+                                 T gc2, T gc3, T dt_ratio, T* u1, T* u2, T* u3, hipStream_t s);
     static hipError_t velprd_rotate(const Geo& g, T* u, T* v, T* w, T* ox, T* oy, T* oz,
                                     const T* h, T r, hipStream_t s);
     static long long reduce_blocks(const Geo& g);

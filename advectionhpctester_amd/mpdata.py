@@ -34,6 +34,7 @@ from ._lib import LIB, MpdataError, check  # noqa: F401  (re-exported)
 
 UPWIND, GAUGE, STANDARD = _lib.UPWIND, _lib.GAUGE, _lib.STANDARD
 PATH_AUTO, PATH_STAGES, PATH_FUSED = _lib.PATH_AUTO, _lib.PATH_STAGES, _lib.PATH_FUSED
+PATH_FAST = _lib.PATH_FAST
 
 
 def field_shape(np_, mp, lp, ih=1, kind=0):
@@ -273,6 +274,18 @@ class Domain:
             self._ctx, self._levels(ox, "ox"), self._levels(oy, "oy"), self._levels(oz, "oz"),
             self._p(u1, 1, "u1"), self._p(u2, 2, "u2"), self._p(u3, 3, "u3"), self._p(h, 0, "h"),
             R(dxi), R(dyi), R(dzi), R(dt), R(dt_ratio), self._stream()))
+
+    def velprd_c(self, ox, oy, oz, oxp, oyp, ozp, h, dxi, dyi, dzi, dt, dt_ratio, u1, u2, u3):
+        """C-grid Courant numbers at n+1/2 from level n (ox, oy, oz) and the
+        previous step's level-n arrays (oxp, oyp, ozp; level n-1 = oxp*h):
+        velprd_traj0 + velprdA_to_C in one pass, levels rotated by swapping
+        the arrays between steps (open walls)."""
+        R = self.real
+        check(self._fn("mpdata_velprd_c")(
+            self._ctx, self._p(ox, 0, "ox"), self._p(oy, 0, "oy"), self._p(oz, 0, "oz"),
+            self._p(oxp, 0, "oxp"), self._p(oyp, 0, "oyp"), self._p(ozp, 0, "ozp"),
+            self._p(h, 0, "h"), R(dxi), R(dyi), R(dzi), R(dt), R(dt_ratio),
+            self._p(u1, 1, "u1"), self._p(u2, 2, "u2"), self._p(u3, 3, "u3"), self._stream()))
 
     def velprdA_to_C(self, ox, oy, oz, u1, u2, u3):
         check(self._fn("mpdata_velprdA_to_C")(

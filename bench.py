@@ -10,6 +10,12 @@ the GPU and resident in HBM before the timed region.
 
 A step is one call of mpdata_step_oop (x_out <- MPDATA(x_in)); the timed loop
 alternates two field buffers, which is how the library's own time loop runs.
+The timed path is the fused kernel's fast build (--path fast: FMA contraction
+and Newton-refined reciprocal division); before the line is printed it is
+checked against the exactly rounded fused step over 50 steps to the 1e-12
+normwise relative tolerance the path must meet, and the exact fused step is
+checked bit for bit against the per-stage kernels and timed as well
+(exact_fused).
 Prints one JSON line:
   value      cell-updates/s (interior cells x steps / seconds)
   roofline   achieved = algorithmic bytes per step (56 B per cell: read x,
@@ -48,7 +54,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, nargs=3, default=(512, 512, 256), metavar=("NP", "MP", "LP"))
     ap.add_argument("--dtype", choices=("f64", "f32"), default="f64")
-    ap.add_argument("--path", choices=("auto", "fused", "stages"), default="auto")
+    ap.add_argument("--path", choices=("fast", "auto", "fused", "stages"), default="fast")
     ap.add_argument("--cpu-steps", type=int, default=None,
                     help="steps of the CPU baseline sample (default 10 through the "
                          "reference build, 2 through the oracle; 0 disables it)")
@@ -115,8 +121,9 @@ def main():
     torch.cuda.set_device(dev)
     case = testdata.tilted_rotation_case(np_, mp_, lp_, cmax=0.8, dtype=tdt, device=dev)
     dom = mpdata.Domain(np_, mp_, lp_, dtype=tdt, device=dev)
-    dom.set_path({"auto": mpdata.PATH_AUTO, "fused": mpdata.PATH_FUSED,
-                  "stages": mpdata.PATH_STAGES}[a.path])
+    paths = {"fast": mpdata.PATH_FAST, "auto": mpdata.PATH_AUTO, "fused": mpdata.PATH_FUSED,
+             "stages": mpdata.PATH_STAGES}
+    dom.set_path(paths[a.path])
     bufs = [case["x"], torch.empty_like(case["x"])]
     bufs[1].copy_(bufs[0])
     args = (case["u1"], case["u2"], case["u3"], case["rhr"], case["h"], case["bcx"], case["bcy"])
@@ -154,11 +161,23 @@ def main():
         if key in t:
             traffic = t[key]["bytes_per_step"]
 
-    # correctness of what was timed: fused and per-stage paths agree bit for bit
-    checked = None
+    # the exactly rounded fused step, timed the same way
+    exact = None
+    if a.path == "fast":
+        dom.set_path(mpdata.PATH_FUSED)
+        s, ms_ex, _ = timed(s)
+        exact = {"ms_per_step": ms_ex / max(a.steps, 1),
+                 "value": cells * a.steps / (ms_ex / 1e3),
+                 "frac": alg_bytes / (ms_ex / max(a.steps, 1) / 1e3) / 1e9 / HBM_PEAK_GBS}
+
+    # correctness of what was timed: the exact fused and per-stage paths agree
+    # bit for bit on one step; the fast path stays within 1e-12 (normwise
+    # relative, interior) of the exact one over 50 steps
+    checked = fast_err = None
     if not a.no_check:
         x0 = bufs[s & 1]
         xf = torch.empty_like(x0)
+        dom.set_path(mpdata.PATH_FUSED)
         dom.step_oop(x0, xf, *args)
         xs = x0.clone()
         dom.set_path(mpdata.PATH_STAGES)
@@ -166,6 +185,20 @@ def main():
         torch.cuda.synchronize()
         checked = bool(torch.equal(xf, xs))
         del xf, xs
+        if a.path == "fast":
+            res = []
+            for p in (mpdata.PATH_FAST, mpdata.PATH_FUSED):
+                dom.set_path(p)
+                xr = x0.clone()
+                dom.timeloop(50, xr, *args)
+                res.append(mpdata.interior(xr))
+            torch.cuda.synchronize()
+            fast_err = float((res[0] - res[1]).abs().max() / res[1].abs().max())
+            del res, xr
+            if not fast_err <= 1e-12:
+                print(f"bench.py: fast path differs from the exact step by {fast_err:.3e} "
+                      "(> 1e-12) after 50 steps", file=sys.stderr)
+                sys.exit(1)
 
     unfused = None
     if not a.no_unfused and a.path != "stages":
@@ -225,12 +258,17 @@ def main():
                         "Courant 0.8, Fourier-mode field k<=8",
             "grid": [np_, mp_, lp_], "variant": "gauge", "iters": 2, "nonos": 1,
             "boundaries": "open", "path": a.path, "parallelism": "single GPU",
+            "arithmetic": ("FP64, FMA contraction and Newton-refined reciprocal division; "
+                           "checked within 1e-12 of the exactly rounded step"
+                           if a.path == "fast" else "FP64, exactly rounded (reference order)"),
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_step": alg_bytes,
                      "duration": "HIP events around the timed steps on the launch stream; "
                                  "a step is the fused kernel plus a halo-copy kernel"},
+        "exact_fused": exact,
+        "max_rel_err_fast_vs_exact_50_steps": fast_err,
         "cpu_baseline": cpu,
         "unfused_stages": unfused,
         "ensemble": ens,
@@ -374,20 +412,33 @@ DENSITY_ARRAYS = 37
 
 
 def configs_bench(steps):
-    """The two configurations that run the per-stage kernels (periodic walls
-    and three iterations are outside the fused kernel):
+    """The periodic-standard and deformational-flow configurations.
       periodic: 64x64x32, periodic in all directions, standard scheme, 2
         iterations, limiter off, solid-body rotation about z at Courant 0.5,
-        Gaussian sphere (testdata.periodic_rotation_case); 100 steps;
+        Gaussian sphere (testdata.periodic_rotation_case); 100 steps; per-stage
+        kernels (periodic walls are outside the fused kernel); 56 B/cell.
       deformational: 256x256x128, time-varying deformational flow at Courant
         0.7 with the velocity predictor every step, gauge, 3 iterations,
-        limiter on, slotted cylinders, FP64 and FP32; a step is the
-        prediction to n+1/2, the MPDATA step and the level rotation.
-    Timed with events on torch's current stream after 2 warm-up steps."""
+        limiter on, slotted cylinders, FP64 and FP32.  A step is the one-pass
+        predictor to n+1/2 (velprd_c, the time levels rotated by swapping the
+        level arrays) and the three-iteration fused step; 80 B/cell in FP64
+        (read x, h, rhr and the A-grid velocity at n and n-1, write x), 40 B in
+        FP32.  ms_per_step excludes the harness's analytic velocity update
+        (level n = U*cos(pi t/T), three elementwise products), which
+        ms_per_step_with_velocity_update includes; stages_ms_per_step is the
+        same step on the per-stage kernels with stored levels and the
+        reference's level rotation.
+    Each entry carries roofline (algorithmic bytes / step time against
+    8 TB/s) and cpu_baseline: the oracle (plain C restatement, one thread;
+    the reference cannot run these options) on a bounded sample of the same
+    workload.  Timed with events on torch's current stream after 2 warm-up
+    steps."""
     import numpy as np
     import torch
 
     from advectionhpctester_amd import mpdata, testdata
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
     dev = torch.device("cuda:0")
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
@@ -400,8 +451,14 @@ def configs_bench(steps):
         torch.cuda.synchronize()
         return ev0.elapsed_time(ev1) / n
 
+    def roof(cells, ms, bpc):
+        gbs = cells * bpc / (ms / 1e3) / 1e9
+        return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_PEAK_GBS, "bytes_per_cell": bpc}
+
     out = {}
     dims = (64, 64, 32)
+    cells1 = dims[0] * dims[1] * dims[2]
     c = testdata.periodic_rotation_case(*dims, seed=11)
     t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in c.items()}
     with mpdata.Domain(*dims, ibc=(1, 1, 1)) as d:
@@ -410,40 +467,103 @@ def configs_bench(steps):
                 d.step(t["x"], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"],
                        variant=mpdata.STANDARD, iters=2, nonos=0)
         ms = timed(periodic, 100)
-    out["periodic_64x64x32_standard"] = {"ms_per_step": ms,
-                                         "value": dims[0] * dims[1] * dims[2] / (ms / 1e3)}
+    orc = Oracle(np.float64)
+    x = c["x"].copy()
+    n_cpu = 100
+    t0 = time.perf_counter()
+    for _ in range(n_cpu):
+        orc.step(2, 2, 0, x, c["u1"], c["u2"], c["u3"], c["rhr"], c["h"], c["bcx"], c["bcy"],
+                 (1, 1, 1), dims + (1,))
+    dt_cpu = time.perf_counter() - t0
+    out["periodic_64x64x32_standard"] = {
+        "ms_per_step": ms, "value": cells1 / (ms / 1e3), "path": "stages",
+        "roofline": roof(cells1, ms, 56),
+        "cpu_baseline": {"value": cells1 * n_cpu / dt_cpu, "unit": "cell-updates/s", "cores": 1,
+                         "kind": "port",
+                         "sample": f"{n_cpu} steps of the same 64x64x32 workload with "
+                                   f"oracle/liboracle_mpdata.so (one thread), {dt_cpu:.1f} s"}}
 
     np_, mp_, lp_ = 256, 256, 128
+    cells2 = np_ * mp_ * lp_
     period = 200
     vel = testdata.deformational_velocity(np_, mp_, lp_, cmax=0.7)
     x0 = testdata.slotted_cylinders(np_, mp_, lp_)
-    for tdt, name in ((torch.float64, "f64"), (torch.float32, "f32")):
+    for tdt, name, bpc in ((torch.float64, "f64", 80), (torch.float32, "f32", 40)):
         U = [torch.from_numpy(np.ascontiguousarray(a)).to(dev, tdt) for a in vel]
-        x = torch.from_numpy(np.ascontiguousarray(x0)).to(dev, tdt)
-        h = torch.ones_like(x)
+        xs = [torch.from_numpy(np.ascontiguousarray(x0)).to(dev, tdt)]
+        xs.append(torch.empty_like(xs[0]))
+        h = torch.ones_like(xs[0])
         bcx = torch.zeros((2, lp_, mp_), dtype=tdt, device=dev)
         bcy = torch.zeros((2, lp_, np_), dtype=tdt, device=dev)
-        lev = [torch.zeros((3,) + tuple(x.shape), dtype=tdt, device=dev) for _ in range(6)]
-        ox, oy, oz, uu, vv, ww = lev
+        lev = [[a.clone() for a in U], [a.clone() for a in U]]  # level n, level n-1
         with mpdata.Domain(np_, mp_, lp_, dtype=tdt) as d:
+            d.set_path(mpdata.PATH_FUSED)
             u1, u2, u3 = (d.zeros(k) for k in (1, 2, 3))
             state = {"n": 0}
 
-            def deform(n):
+            def deform(n, update=True):
                 for _ in range(n):
-                    f = testdata.reversal_factor(state["n"], period)
-                    for o, a in zip((ox, oy, oz), U):
-                        o[0] = a * f
-                        if state["n"] == 0:
-                            o[2] = o[0] * h
-                    d.velprd_traj0(ox, oy, oz, u1, u2, u3, h, 1.0, 1.0, 1.0, 1.0, 1.0)
-                    d.step(x, u1, u2, u3, h, h, bcx, bcy, variant=mpdata.GAUGE, iters=3, nonos=1)
-                    d.velprd_rotate(uu, vv, ww, ox, oy, oz, h, 1.0)
+                    k = state["n"]
+                    cur, prev = lev[k & 1], lev[(k + 1) & 1]
+                    if update:
+                        f = testdata.reversal_factor(k, period)
+                        for o, a in zip(cur, U):
+                            torch.mul(a, f, out=o)
+                    d.velprd_c(*cur, *prev, h, 1.0, 1.0, 1.0, 1.0, 1.0, u1, u2, u3)
+                    d.step_oop(xs[k & 1], xs[(k + 1) & 1], u1, u2, u3, h, h, bcx, bcy,
+                               variant=mpdata.GAUGE, iters=3, nonos=1)
                     state["n"] += 1
-            ms = timed(deform, steps)
-        out[f"deformational_256x256x128_gauge3_{name}"] = {
-            "ms_per_step": ms, "value": np_ * mp_ * lp_ / (ms / 1e3)}
-        del U, x, h, lev, ox, oy, oz, uu, vv, ww
+            ms_upd = timed(deform, steps)
+            ms = timed(lambda n: deform(n, update=False), steps)
+
+            # the per-stage path with stored levels and the reference's rotation
+            d.set_path(mpdata.PATH_STAGES)
+            L = [torch.zeros((3,) + tuple(h.shape), dtype=tdt, device=dev) for _ in range(6)]
+            ox, oy, oz, uu, vv, ww = L
+            for o, a in zip((ox, oy, oz), U):
+                o[0] = a
+                o[2] = a * h
+            xst = xs[0].clone()
+
+            def stages(n):
+                for _ in range(n):
+                    d.velprd_traj0(ox, oy, oz, u1, u2, u3, h, 1.0, 1.0, 1.0, 1.0, 1.0)
+                    d.step(xst, u1, u2, u3, h, h, bcx, bcy, variant=mpdata.GAUGE, iters=3, nonos=1)
+                    d.velprd_rotate(uu, vv, ww, ox, oy, oz, h, 1.0)
+            ms_st = timed(stages, max(steps // 2, 2))
+            del L, ox, oy, oz, uu, vv, ww, xst
+        entry = {"ms_per_step": ms, "value": cells2 / (ms / 1e3), "path": "fused",
+                 "ms_per_step_with_velocity_update": ms_upd, "stages_ms_per_step": ms_st,
+                 "roofline": roof(cells2, ms, bpc)}
+        if name == "f64":
+            npdt = np.float64
+            orc = Oracle(npdt)
+            host = {"x": np.ascontiguousarray(x0.astype(npdt))}
+            S = host["x"].shape
+            Lh = [np.zeros((3,) + S, npdt) for _ in range(6)]
+            uh = [np.zeros(shape_c, npdt) for shape_c in
+                  ((lp_ + 2, mp_ + 2, np_ + 3), (lp_ + 2, mp_ + 3, np_ + 2), (lp_ + 3, mp_ + 2, np_ + 2))]
+            hh = np.ones(S, npdt)
+            bx, by = np.zeros((2, lp_, mp_), npdt), np.zeros((2, lp_, np_), npdt)
+            for o, a in zip(Lh[:3], vel):
+                o[0] = a
+                o[2] = a * hh
+            n_cpu = 2
+            t0 = time.perf_counter()
+            for _ in range(n_cpu):
+                orc.velprd_traj0(*Lh[:3], *uh, hh, 1.0, 1.0, 1.0, 1.0, 1.0, (0, 0, 0),
+                                 (np_, mp_, lp_, 1))
+                orc.step(1, 3, 1, host["x"], *uh, hh, hh, bx, by, (0, 0, 0), (np_, mp_, lp_, 1))
+                orc.velprd_rotate(*Lh[3:], *Lh[:3], hh, 1.0, (np_, mp_, lp_, 1))
+            dt_cpu = time.perf_counter() - t0
+            entry["cpu_baseline"] = {
+                "value": cells2 * n_cpu / dt_cpu, "unit": "cell-updates/s", "cores": 1,
+                "kind": "port",
+                "sample": f"{n_cpu} step(s) of the same 256x256x128 FP64 workload (predictor, "
+                          f"3-iteration gauge step, level rotation) with oracle/liboracle_mpdata.so "
+                          f"(one thread), {dt_cpu:.1f} s"}
+        out[f"deformational_256x256x128_gauge3_{name}"] = entry
+        del U, xs, h, lev
         torch.cuda.empty_cache()
     out["steps"] = steps
     return out

@@ -57,6 +57,12 @@ extern "C" {
 #define MPDATA_PATH_AUTO 0   /* fused kernel when it supports the options */
 #define MPDATA_PATH_STAGES 1 /* one kernel per reference stage            */
 #define MPDATA_PATH_FUSED 2  /* single-pass fused kernel (error if unsupported) */
+/* The fused kernel's fast build (FMA contraction, Newton-refined reciprocal
+ * division) for the single-domain step, step_oop and time loop; not
+ * bit-identical to the reference (a few ulps per step, about 1e-15
+ * normwise after 50 steps at 512x512x256); error if unsupported.  The other
+ * entry points treat it as MPDATA_PATH_AUTO. */
+#define MPDATA_PATH_FAST 3
 
 typedef struct mpdata_ctx mpdata_ctx;
 
@@ -225,6 +231,23 @@ int mpdata_velprd_rotate_f64(mpdata_ctx *ctx, double *u, double *v, double *w, d
 int mpdata_velprd_rotate_f32(mpdata_ctx *ctx, float *u, float *v, float *w, float *ox, float *oy,
                              float *oz, const float *h, float dt_ratio, void *stream);
 
+/* velprd_traj0 followed by velprdA_to_C (module_velprd.f90:15-89, 166-364)
+ * in one pass, open walls, for a time loop that rotates the time levels by
+ * swapping pointers: ox, oy, oz are level n (one 3-D array each) and oxp,
+ * oyp, ozp the previous step's level-n arrays, which this call multiplies by
+ * h to form level n-1 as the reference's rotation stores it (ox(.,2) =
+ * ox(.,0)*h, velprd_driver.f90:254-270; h must be that rotation's h).
+ * u1..u3 are bit-identical to velprd_traj0 + velprdA_to_C on the stored
+ * levels; the A-grid level n+1/2 is not stored.  No counterpart of its own
+ * in the reference (it fuses two of its subroutines and its rotation). */
+int mpdata_velprd_c_f64(mpdata_ctx *ctx, const double *ox, const double *oy, const double *oz,
+                        const double *oxp, const double *oyp, const double *ozp,
+                        const double *h, double dxi, double dyi, double dzi, double dt,
+                        double dt_ratio, double *u1, double *u2, double *u3, void *stream);
+int mpdata_velprd_c_f32(mpdata_ctx *ctx, const float *ox, const float *oy, const float *oz,
+                        const float *oxp, const float *oyp, const float *ozp, const float *h,
+                        float dxi, float dyi, float dzi, float dt, float dt_ratio, float *u1,
+                        float *u2, float *u3, void *stream);
 /* velprd_traj0_adapt_dt (module_velprd.f90:91-164): the extrapolation of
  * velprd_traj0 into level 1 without the C-grid averaging. */
 int mpdata_velprd_traj0_adapt_dt_f64(mpdata_ctx *ctx, double *ox, double *oy, double *oz,
