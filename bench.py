@@ -30,8 +30,8 @@ Sub-objects: unfused_stages (the per-stage kernels), ensemble (64 uneven
 tracers in one persistent launch / HIP graph), multifield (eight fields on one
 grid), density (the density step with the velocity predictor), configs (the
 periodic-standard and deformational-flow configurations, per-stage kernels),
-adjoint (forward+backward against one forward).  --density-only prints the density
-object alone.
+adjoint (forward+backward against one forward).  --density-only and
+--configs-only print the density or configs object alone.
 """
 import argparse
 import json
@@ -71,6 +71,9 @@ def parse():
     ap.add_argument("--density-only", action="store_true",
                     help="run only the density-step measurement (--steps calls) and print its "
                          "JSON object")
+    ap.add_argument("--configs-only", action="store_true",
+                    help="run only the periodic-standard and deformational-flow configurations "
+                         "(--steps steps each) and print their JSON object")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the periodic-standard and deformational-flow configurations "
                          "(per-stage kernels)")
@@ -110,6 +113,9 @@ def main():
         return
     if a.density_only:
         print(json.dumps(density_bench(a.steps)), flush=True)
+        return
+    if a.configs_only:
+        print(json.dumps(configs_bench(a.steps)), flush=True)
         return
     import torch
 
