@@ -94,6 +94,17 @@ def _load():
     lib.mpdata_dev_free.argtypes = [_P]
     lib.mpdata_copy.argtypes = [_P, _P, _LL, _I]
     lib.mpdata_sync.argtypes = [_P]
+    lib.mpdata_ttini.argtypes = []
+    lib.mpdata_ttbeg.argtypes = [_I, _P]
+    lib.mpdata_ttend.argtypes = [_I, _P]
+    lib.mpdata_ttget.argtypes = [_I, ctypes.POINTER(_I), ctypes.POINTER(_D)]
+    lib.mpdata_ttprt.argtypes = []
+    lib.mpdata_serializer_open.argtypes = [ctypes.c_char_p]
+    lib.mpdata_serializer_close.argtypes = []
+    for sfx in ("_f64", "_f32"):
+        fn = getattr(lib, "mpdata_serialize_field" + sfx)
+        fn.argtypes = [ctypes.c_char_p, ctypes.c_char_p, _P, _I, _I, _I, _P]
+        fn.restype = _I
     for sfx, real in (("_f64", _D), ("_f32", _F)):
         for name, args in _proto(real).items():
             fn = getattr(lib, name + sfx)
@@ -115,7 +126,10 @@ def check(rc):
 def exported_symbols():
     """Names of the functions include/mpdata_hip.h declares."""
     names = ["mpdata_version", "mpdata_last_error", "mpdata_ctx_create", "mpdata_ctx_destroy",
-             "mpdata_ctx_set_path", "mpdata_ctx_scratch_bytes", "upwind_c_timeloop"]
+             "mpdata_ctx_set_path", "mpdata_ctx_scratch_bytes", "upwind_c_timeloop",
+             "mpdata_serializer_open", "mpdata_serializer_close", "mpdata_serialize_field_f64",
+             "mpdata_serialize_field_f32", "mpdata_ttini", "mpdata_ttbeg", "mpdata_ttend",
+             "mpdata_ttget", "mpdata_ttprt"]
     for base in _proto(_D):
         names += [base + "_f64", base + "_f32"]
     return names

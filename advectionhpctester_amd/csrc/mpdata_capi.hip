@@ -15,6 +15,7 @@
 #include "mpdata_fused.hpp"
 #include "mpdata_stages.hpp"
 #include "mpdata_status.hpp"
+#include "mpdata_timers.hpp"
 
 using namespace mpdata;
 
@@ -192,7 +193,10 @@ int step_stages(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T*
     int rc;
     if ((rc = ensure(c, &S.xant, SCAL, sizeof(T)))) return rc;
     T* xant = (T*)S.xant;
-    if ((rc = upwind_impl<T>(c, u1, u2, u3, x, xant, rhr, h, bcx, bcy, s))) return rc;
+    tt_beg(16, s);  // upwind3d-gpubc.f90:87-200
+    rc = upwind_impl<T>(c, u1, u2, u3, x, xant, rhr, h, bcx, bcy, s);
+    tt_end(16, s);
+    if (rc) return rc;
     if (variant == MPDATA_UPWIND || iters <= 1) {
         HIPCHK(Stages<T>::copy_interior(c->g, xant, x, s));
         return MPDATA_OK;
@@ -213,9 +217,13 @@ int step_stages(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T*
         const T* a1 = it == 2 ? u1 : adv[0];
         const T* a2 = it == 2 ? u2 : adv[1];
         const T* a3 = it == 2 ? u3 : adv[2];
+        // antidiff3d-gauge-gpubc.f90:62-434 / antidiff3d-standard-gpubc.f90:54-462
+        const int tid = variant == MPDATA_GAUGE ? 51 : 53;
+        tt_beg(tid, s);
         rc = stage_pass<T>(c, variant, nonos, a1, a2, a3, psi, x, last ? x : nxt, rhr, h,
                            last ? nullptr : lim[0], last ? nullptr : lim[1],
                            last ? nullptr : lim[2], s);
+        tt_end(tid, s);
         if (rc) return rc;
         if (!last) {
             for (int d = 0; d < 3; d++) {
@@ -522,8 +530,13 @@ long long mpdata_ctx_scratch_bytes(const mpdata_ctx* ctx) { return ctx ? ctx->he
             return fail(MPDATA_EINVAL,                                                           \
                         "legacyoptmode %d / t_adv %d: advec.f90 accepts 4 with t_adv 1|2, or 6", \
                         legacyoptmode, t_adv);                                                   \
-        return step_impl<T>(ctx, variant, 2, 1, x, uadv, vadv, wadv, rhr, rhoadv, x_lr, x_bt,    \
-                            (hipStream_t)stream);                                                \
+        /* the two-step driver's timer, advec.f90:183-240 (none in mode 6) */                   \
+        const int tid = legacyoptmode == 6 ? 0 : (t_adv == 1 ? 55 : 57);                        \
+        if (tid) tt_beg(tid, (hipStream_t)stream);                                               \
+        rc = step_impl<T>(ctx, variant, 2, 1, x, uadv, vadv, wadv, rhr, rhoadv, x_lr, x_bt,      \
+                          (hipStream_t)stream);                                                  \
+        if (tid) tt_end(tid, (hipStream_t)stream);                                               \
+        return rc;                                                                               \
     }                                                                                            \
     int mpdata_step##SFX(mpdata_ctx* ctx, int variant, int iters, int nonos, T* x, const T* u1,  \
                          const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,       \

@@ -3,7 +3,11 @@
 ! th, u, v, w, tht, pstr, qv and qc on a 30x22x14 open domain with one
 ! velocity, rhr and rhoadv, then each field again through mpdata_step with
 ! the variant CE_advection gives it.  Prints the largest difference per
-! field and "CE check: PASS" when every field is identical.
+! field and "CE check: PASS" when every field is identical.  With a
+! directory as the first argument, CE_advection runs with the reference's
+! serialization flags (savepoints mysnapshot.run-in / run-out in that
+! directory), and the program also writes the inputs and results of its own
+! per-field steps at savepoints check.in / check.out.
 PROGRAM mpdata_ce_check
    USE iso_c_binding
    USE mpdata_hip
@@ -16,6 +20,15 @@ PROGRAM mpdata_ce_check
    TYPE(c_ptr) :: x(nf), y(nf), bx(nf), by(nf), u1, u2, u3, rhr, h
    INTEGER :: f, i, j, k, variant
    REAL(euwp) :: d, worst
+   CHARACTER(len=512) :: serdir
+   LOGICAL :: ser
+   CHARACTER(len=4), PARAMETER :: names(nf) = (/ 'th  ', 'u   ', 'v   ', 'w   ', 'tht ', 'pstr', &
+                                                 'qv  ', 'qc  ' /)
+
+   serdir = ''
+   IF (COMMAND_ARGUMENT_COUNT() >= 1) CALL GET_COMMAND_ARGUMENT(1, serdir)
+   ser = LEN_TRIM(serdir) > 0
+   IF (ser) CALL mpdata_serializer_open(TRIM(serdir))
 
    CALL mpdata_init(np, mp, lp, ih, 0, 0, 0)
    ns = INT(np + 2 * ih, c_long_long) * (mp + 2 * ih) * (lp + 2 * ih)
@@ -86,7 +99,12 @@ PROGRAM mpdata_ce_check
       CALL mpdata_put(by(f), hby, nby)
    END DO
 
-   CALL CE_advection(.FALSE., 0, 0, 0, 0, 0, 0, np, mp, lp, ih, .FALSE., .FALSE., &
+   IF (ser) THEN
+      DO f = 1, nf
+         CALL mpdata_serialize('check.in', TRIM(names(f)), y(f), np + 2*ih, mp + 2*ih, lp + 2*ih)
+      END DO
+   END IF
+   CALL CE_advection(.FALSE., 0, 0, 0, 0, 0, 0, np, mp, lp, ih, ser, ser, &
                      rhr, h, u1, u2, u3, &
                      x(1), bx(1), by(1), x(2), bx(2), by(2), x(3), bx(3), by(3), &
                      x(4), bx(4), by(4), x(5), bx(5), by(5), x(6), bx(6), by(6), &
@@ -96,6 +114,12 @@ PROGRAM mpdata_ce_check
       CALL mpdata_step(variant, 2, .TRUE., y(f), u1, u2, u3, rhr, h, bx(f), by(f))
    END DO
    CALL mpdata_sync()
+   IF (ser) THEN
+      DO f = 1, nf
+         CALL mpdata_serialize('check.out', TRIM(names(f)), y(f), np + 2*ih, mp + 2*ih, lp + 2*ih)
+      END DO
+      CALL mpdata_serializer_close()
+   END IF
    worst = 0.0_euwp
    DO f = 1, nf
       CALL mpdata_get(a, x(f), ns)

@@ -45,6 +45,8 @@ MODULE mpdata_hip
    PUBLIC :: upwind3d_gpubc, antidiff3d_gauge_gpubc, antidiff3d_standard_gpubc, advec_dwarf
    PUBLIC :: mpdata_step, mpdata_timeloop, velprdA_to_C, compute_courants, norm_sums
    PUBLIC :: mpdata_version, CE_advection, mpdata3d_rho_gauge_gpubc, velprd_driver
+   PUBLIC :: mpdata_serializer_open, mpdata_serializer_close, mpdata_serialize
+   PUBLIC :: ttini, ttbeg, ttend, ttprt
 
    ! the module-global context (the reference's module-global scratch) and
    ! the domain it was created for (np, mp, lp, ih, ibcx, ibcy, ibcz)
@@ -52,6 +54,43 @@ MODULE mpdata_hip
    INTEGER, SAVE :: ctx_dims(7) = 0
 
    INTERFACE
+      FUNCTION c_serializer_open(dir) BIND(C, name="mpdata_serializer_open")
+         IMPORT :: c_int, c_char
+         CHARACTER(KIND=c_char), DIMENSION(*), INTENT(IN) :: dir
+         INTEGER(c_int) :: c_serializer_open
+      END FUNCTION
+      FUNCTION c_serializer_close() BIND(C, name="mpdata_serializer_close")
+         IMPORT :: c_int
+         INTEGER(c_int) :: c_serializer_close
+      END FUNCTION
+      FUNCTION c_serialize_field(savepoint, name, field, n1, n2, n3, stream) &
+            BIND(C, name="mpdata_serialize_field"//MP_SFX)
+         IMPORT :: c_int, c_char, c_ptr
+         CHARACTER(KIND=c_char), DIMENSION(*), INTENT(IN) :: savepoint, name
+         TYPE(c_ptr), VALUE :: field, stream
+         INTEGER(c_int), VALUE :: n1, n2, n3
+         INTEGER(c_int) :: c_serialize_field
+      END FUNCTION
+      FUNCTION c_ttini() BIND(C, name="mpdata_ttini")
+         IMPORT :: c_int
+         INTEGER(c_int) :: c_ttini
+      END FUNCTION
+      FUNCTION c_ttbeg(id, stream) BIND(C, name="mpdata_ttbeg")
+         IMPORT :: c_int, c_ptr
+         INTEGER(c_int), VALUE :: id
+         TYPE(c_ptr), VALUE :: stream
+         INTEGER(c_int) :: c_ttbeg
+      END FUNCTION
+      FUNCTION c_ttend(id, stream) BIND(C, name="mpdata_ttend")
+         IMPORT :: c_int, c_ptr
+         INTEGER(c_int), VALUE :: id
+         TYPE(c_ptr), VALUE :: stream
+         INTEGER(c_int) :: c_ttend
+      END FUNCTION
+      FUNCTION c_ttprt() BIND(C, name="mpdata_ttprt")
+         IMPORT :: c_int
+         INTEGER(c_int) :: c_ttprt
+      END FUNCTION
       FUNCTION c_version() BIND(C, name="mpdata_version")
          IMPORT :: c_ptr
          TYPE(c_ptr) :: c_version
@@ -399,6 +438,49 @@ CONTAINS
    ! take the antidiffusive pass the reference calls mpdatm3d_antidiff, th
    ! and the moist fields mpdata3d_antidiff (:145-155, :270-304).  lupdatemulti
    ! and iflip have no effect on one process without poles.
+   ! Timers keyed by the reference's ids (common/mpi_parallel.f90:7088-7447),
+   ! HIP events on the default stream; ttini turns them on and resets them.
+   SUBROUTINE ttini()
+      CALL mpdata_check(c_ttini(), 'ttini')
+   END SUBROUTINE ttini
+
+   SUBROUTINE ttbeg(id)
+      INTEGER, INTENT(IN) :: id
+      CALL mpdata_check(c_ttbeg(INT(id, c_int), c_null_ptr), 'ttbeg')
+   END SUBROUTINE ttbeg
+
+   SUBROUTINE ttend(id)
+      INTEGER, INTENT(IN) :: id
+      CALL mpdata_check(c_ttend(INT(id, c_int), c_null_ptr), 'ttend')
+   END SUBROUTINE ttend
+
+   SUBROUTINE ttprt()
+      FLUSH (6)
+      CALL mpdata_check(c_ttprt(), 'ttprt')
+   END SUBROUTINE ttprt
+
+   ! Serialization savepoints (the reference's serialbox calls under
+   ! SERIALIZE): fields written as <dir>/<savepoint>__<name>.npy with an index
+   ! line in <dir>/savepoints.txt.
+   SUBROUTINE mpdata_serializer_open(dir)
+      CHARACTER(*), INTENT(IN) :: dir
+      CALL mpdata_check(c_serializer_open(TRIM(dir)//c_null_char), 'mpdata_serializer_open')
+   END SUBROUTINE mpdata_serializer_open
+
+   SUBROUTINE mpdata_serializer_close()
+      CALL mpdata_check(c_serializer_close(), 'mpdata_serializer_close')
+   END SUBROUTINE mpdata_serializer_close
+
+   ! one field (device handle or host C_LOC) of the Fortran shape n1 x n2 x n3
+   SUBROUTINE mpdata_serialize(savepoint, name, field, n1, n2, n3)
+      CHARACTER(*), INTENT(IN) :: savepoint, name
+      TYPE(c_ptr), INTENT(IN) :: field
+      INTEGER, INTENT(IN) :: n1, n2, n3
+      CALL mpdata_check(c_serialize_field(TRIM(savepoint)//c_null_char, TRIM(name)//c_null_char, &
+                                          field, INT(n1, c_int), INT(n2, c_int), INT(n3, c_int), &
+                                          c_null_ptr), 'mpdata_serialize')
+   END SUBROUTINE mpdata_serialize
+
    SUBROUTINE CE_advection(lupdatemulti, mpdata_alg_choice, liner, ipoles, ibcx, ibcy, ibcz, &
                            np, mp, lp, ih, do_serialization_in, do_serialization_out, &
                            rhr, rhoadv, uadv, vadv, wadv, &
@@ -438,13 +520,49 @@ CONTAINS
       IF (PRESENT(qi)) CALL add(qi, qi_lr, qi_bt, 0)
       IF (PRESENT(qs)) CALL add(qs, qs_lr, qs_bt, 0)
       IF (PRESENT(qg)) CALL add(qg, qg_lr, qg_bt, 0)
-      IF (lupdatemulti .OR. PRESENT(do_serialization_in) .OR. PRESENT(do_serialization_out)) THEN
-         ! no effect here (one process, no serialization)
+      IF (lupdatemulti) THEN
+         ! no effect here (one process)
       ENDIF
+      ! the reference's savepoints (advection_CEtimestep_twostep_GPUBC.f90:
+      ! 122-142, 355-375): the inputs at mysnapshot.run-in, the results at
+      ! mysnapshot.run-out, in the open serializer (mpdata_serializer_open)
+      IF (PRESENT(do_serialization_in)) THEN
+         IF (do_serialization_in) CALL savepoint('mysnapshot.run-in', 'uadv', 'vadv', 'wadv')
+      END IF
       CALL mpdata_check(c_ce_advection(ctx, INT(mpdata_alg_choice, c_int), INT(liner, c_int), &
                                        INT(n, c_int), role, x, xl, xb, uadv, vadv, wadv, rhr, &
                                        rhoadv, c_null_ptr), 'CE_advection')
+      IF (PRESENT(do_serialization_out)) THEN
+         IF (do_serialization_out) CALL savepoint('mysnapshot.run-out', 'u1', 'u2', 'u3')
+      END IF
    CONTAINS
+      SUBROUTINE savepoint(sp, n1, n2, n3)
+         CHARACTER(*), INTENT(IN) :: sp, n1, n2, n3
+         CALL mpdata_serialize(sp, n1, uadv, np + 1 + 2*ih, mp + 2*ih, lp + 2*ih)
+         CALL mpdata_serialize(sp, n2, vadv, np + 2*ih, mp + 1 + 2*ih, lp + 2*ih)
+         CALL mpdata_serialize(sp, n3, wadv, np + 2*ih, mp + 2*ih, lp + 1 + 2*ih)
+         IF (sp == 'mysnapshot.run-in') &
+            CALL mpdata_serialize(sp, 'rhoadv', rhoadv, np + 2*ih, mp + 2*ih, lp + 2*ih)
+         CALL field3(sp, 'u', u, u_lr, u_bt)
+         CALL field3(sp, 'v', v, v_lr, v_bt)
+         CALL field3(sp, 'w', w, w_lr, w_bt)
+         CALL field3(sp, 'th', th, th_lr, th_bt)
+         CALL field3(sp, 'tht', tht, tht_lr, tht_bt)
+         CALL field3(sp, 'pstr', pstr, pstr_lr, pstr_bt)
+         IF (PRESENT(qv)) CALL field3(sp, 'qv', qv, qv_lr, qv_bt)
+         IF (PRESENT(qc)) CALL field3(sp, 'qc', qc, qc_lr, qc_bt)
+         IF (PRESENT(qr)) CALL field3(sp, 'qr', qr, qr_lr, qr_bt)
+         IF (PRESENT(qi)) CALL field3(sp, 'qi', qi, qi_lr, qi_bt)
+         IF (PRESENT(qs)) CALL field3(sp, 'qs', qs, qs_lr, qs_bt)
+         IF (PRESENT(qg)) CALL field3(sp, 'qg', qg, qg_lr, qg_bt)
+      END SUBROUTINE savepoint
+      SUBROUTINE field3(sp, nm, f, f_lr, f_bt)
+         CHARACTER(*), INTENT(IN) :: sp, nm
+         TYPE(c_ptr), INTENT(IN) :: f, f_lr, f_bt
+         CALL mpdata_serialize(sp, nm, f, np + 2*ih, mp + 2*ih, lp + 2*ih)
+         CALL mpdata_serialize(sp, nm//'_lr', f_lr, mp, lp, 2)
+         CALL mpdata_serialize(sp, nm//'_bt', f_bt, np, lp, 2)
+      END SUBROUTINE field3
       SUBROUTINE add(f, f_lr, f_bt, r)
          TYPE(c_ptr), INTENT(IN) :: f, f_lr, f_bt
          INTEGER, INTENT(IN) :: r

@@ -354,6 +354,37 @@ int mpdata_compute_courants_noalloc_f32(const float *ox, const float *oy, const 
                                         const float *h, int ntimelevel, float dxi, float dyi,
                                         float dzi, float dt, int np, int mp, int lp, int ih,
                                         double *out, void *stream);
+/* ---- timers ------------------------------------------------------------------
+ * Keyed by the reference's timer ids (common/mpi_parallel.f90:7088-7447:
+ * ttini/ttbeg/ttend/ttprt), measured with HIP events on `stream`.  Off until
+ * mpdata_ttini (which also resets them); then the library itself records
+ * 55/57 (two-step gauge/standard driver, advec.f90:183-240) in
+ * mpdata_advec_dwarf_*, and 16 (upwind) and 51/53 (gauge/standard
+ * antidiffusive pass) around the per-stage passes; a host adds its own ids
+ * (the dwarf's 300 around each advec_dwarf call).  mpdata_ttget returns an
+ * id's call count and summed seconds; mpdata_ttprt prints the reference's
+ * table (one process: average = maximum = minimum) and its sorted list.
+ * Both synchronize the recorded events. */
+int mpdata_ttini(void);
+int mpdata_ttbeg(int id, void *stream);
+int mpdata_ttend(int id, void *stream);
+int mpdata_ttget(int id, int *calls, double *seconds);
+int mpdata_ttprt(void);
+/* ---- serialization savepoints --------------------------------------------
+ * The analogue of the reference's serialbox calls
+ * (advection_CEtimestep_twostep_GPUBC.f90:122-142, 355-375 under SERIALIZE):
+ * open a serializer directory, then write named fields at named savepoints.
+ * Each field becomes <dir>/<savepoint>__<name>.npy (NumPy format, Fortran
+ * order, shape n1 x n2 x n3, no pickled objects) and one line of
+ * <dir>/savepoints.txt ("<savepoint> <name> <file> <f8|f4> <n1> <n2> <n3>").
+ * Fields may be device or host pointers; device fields are copied on
+ * `stream`, which is synchronized.  Names: [A-Za-z0-9._-], 1-200 chars. */
+int mpdata_serializer_open(const char *dir);
+int mpdata_serializer_close(void);
+int mpdata_serialize_field_f64(const char *savepoint, const char *name, const double *field,
+                               int n1, int n2, int n3, void *stream);
+int mpdata_serialize_field_f32(const char *savepoint, const char *name, const float *field,
+                               int n1, int n2, int n3, void *stream);
 /* Interior sums for the dwarf's norms (advection_test_setup.f90:472-492,
  * 595-617): out[4] (host) = sum x^2, sum x0^2, sum (x0-x)^2, sum x
  * (accumulated in FP64).  x0 may be NULL (its terms are then 0). */

@@ -100,6 +100,48 @@ def test_fortran_dwarf_standard_variant(tmp_path, orc64):
     assert np.array_equal(snaps[-1], x[1:-1, 1:-1, 1:-1])
 
 
+def _timer_rows(out):
+    """{item id: (label, calls, seconds)} of the timer tables printed."""
+    rows = {}
+    for line in out.splitlines():
+        p = line.split()
+        if len(p) == 7 and p[0] == "0" and p[1].isdigit() and p[3].isdigit():
+            rows.setdefault(int(p[1]), (p[2], int(p[3]), float(p[4])))
+    return rows
+
+
+def test_fortran_dwarf_size_sweep_and_timers(tmp_path, orc64):
+    """The reference's size sweep (n = 59*iscale, dt/iscale): iscale 2 on
+    118^3, 20 steps, bit-identical to the oracle on the same initial state;
+    each size prints the reference's timer table with advec_dwarf (item 300)
+    and its two-step driver (55) called once per step."""
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is not built (make -C advectionhpctester_amd/fortran)")
+    dump = str(tmp_path / "x2.bin")
+    print("fortran dwarf sweep iscale 2..2, 20 steps", flush=True)
+    r = subprocess.run([EXE, "20", dump, "1", "2:2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "iscale 2, 20 steps on 118 x 118 x 118" in r.stdout, r.stdout
+    rows = _timer_rows(r.stdout)
+    assert rows[300][:2] == ("eulagdwarf", 20), r.stdout
+    assert rows[55][:2] == ("twos_gau_gpubc", 20), r.stdout
+    assert rows[300][2] >= rows[55][2] > 0.0
+    snaps = np.fromfile(dump, dtype=np.float64).reshape(-1, 118, 118, 118)
+    from advectionhpctester_amd import testdata
+    c = testdata.dwarf_setup(iscale=2)
+    x = c.x.copy()
+    dims = (c.n, c.n, c.n, 1)
+    for _ in range(20):
+        orc64.step(GAUGE, 2, 1, x, c.u1, c.u2, c.u3, c.rhr, c.h, c.bcx, c.bcy, (0, 0, 0), dims)
+    assert np.array_equal(snaps[-1], x[1:-1, 1:-1, 1:-1])
+    # two sizes in one run: both tables, the second on 118^3
+    r = subprocess.run([EXE, "3", "-", "1", "1:2"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("Dwarf statistics summary") == 2
+    assert "iscale 1, 3 steps on 59 x 59 x 59" in r.stdout
+
+
 @pytest.mark.parametrize("build", ["build", "build32"])
 def test_fortran_ce_advection_equals_single_steps(build):
     """CE_advection with th, u, v, w, tht, pstr, qv, qc (five gauge, three
