@@ -129,7 +129,7 @@ def exported_symbols():
              "mpdata_ctx_set_path", "mpdata_ctx_scratch_bytes", "upwind_c_timeloop",
              "mpdata_serializer_open", "mpdata_serializer_close", "mpdata_serialize_field_f64",
              "mpdata_serialize_field_f32", "mpdata_ttini", "mpdata_ttbeg", "mpdata_ttend",
-             "mpdata_ttget", "mpdata_ttprt"]
+             "mpdata_ttget", "mpdata_ttprt", "mpdata_addr", "mpdata_is_device_ptr"]
     for base in _proto(_D):
         names += [base + "_f64", base + "_f32"]
     return names

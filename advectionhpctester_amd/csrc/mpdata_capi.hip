@@ -430,6 +430,17 @@ extern "C" {
 
 const char* mpdata_version(void) { return "mpdata_hip 0.1 (gfx950)"; }
 
+const void* mpdata_addr(const void* p) { return p; }
+
+int mpdata_is_device_ptr(const void* p) {
+    if (!p) return 0;
+    hipPointerAttribute_t attr;
+    const bool dev = hipPointerGetAttributes(&attr, p) == hipSuccess &&
+                     (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged);
+    (void)hipGetLastError();
+    return dev ? 1 : 0;
+}
+
 const char* mpdata_last_error(void) { return mpdata::last_error(); }
 
 int mpdata_ctx_create(int np, int mp, int lp, int ih, int ibcx, int ibcy, int ibcz,

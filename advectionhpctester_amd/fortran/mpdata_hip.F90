@@ -46,7 +46,7 @@ MODULE mpdata_hip
    PUBLIC :: mpdata_step, mpdata_timeloop, velprdA_to_C, compute_courants, norm_sums
    PUBLIC :: mpdata_version, CE_advection, mpdata3d_rho_gauge_gpubc, velprd_driver
    PUBLIC :: mpdata_serializer_open, mpdata_serializer_close, mpdata_serialize
-   PUBLIC :: ttini, ttbeg, ttend, ttprt
+   PUBLIC :: ttini, ttbeg, ttend, ttprt, mpdata_ensure_domain
 
    ! the module-global context (the reference's module-global scratch) and
    ! the domain it was created for (np, mp, lp, ih, ibcx, ibcy, ibcz)
@@ -318,6 +318,13 @@ CONTAINS
                                      INT(ibcz, c_int), INT(euwp, c_int), ctx), 'mpdata_init')
       ctx_dims = (/ np, mp, lp, ih, ibcx, ibcy, ibcz /)
    END SUBROUTINE mpdata_init
+
+   ! mpdata_init unless the context already holds this domain
+   SUBROUTINE mpdata_ensure_domain(np, mp, lp, ih, ibcx, ibcy, ibcz)
+      INTEGER, INTENT(IN) :: np, mp, lp, ih, ibcx, ibcy, ibcz
+      IF (c_associated(ctx) .AND. ALL(ctx_dims == (/ np, mp, lp, ih, ibcx, ibcy, ibcz /))) RETURN
+      CALL mpdata_init(np, mp, lp, ih, ibcx, ibcy, ibcz)
+   END SUBROUTINE mpdata_ensure_domain
 
    SUBROUTINE mpdata_finalize()
       IF (c_associated(ctx)) CALL mpdata_check(c_ctx_destroy(ctx), 'mpdata_finalize')

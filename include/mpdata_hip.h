@@ -69,6 +69,11 @@ typedef struct mpdata_ctx mpdata_ctx;
 /* ---- library / context ------------------------------------------------ */
 
 const char *mpdata_version(void);
+/* Helpers of the Fortran drop-ins with array dummies: the address of an
+ * array passed by reference, and whether a pointer is device or managed
+ * memory (1) or host memory (0).  No counterparts in the reference. */
+const void *mpdata_addr(const void *p);
+int mpdata_is_device_ptr(const void *p);
 const char *mpdata_last_error(void);
 
 /* Creates the per-domain context (scratch for the antidiffusive passes) on
