@@ -260,13 +260,19 @@ int use_fused(mpdata_ctx* c, int variant, int iters, int nonos, bool* fused) {
     return MPDATA_OK;
 }
 
-// One fused step out of place: the fast build under MPDATA_PATH_FAST, else
-// the exactly rounded one.  Three iterations keep the second iterate and the
-// second pass's limited velocities in the context's scratch.
+// One fused step out of place: the gauge variant on the fast build under
+// MPDATA_PATH_FAST, else the exactly rounded one; the standard variant on its
+// own (exactly rounded) kernel.  Three iterations keep the second iterate and
+// the second pass's limited velocities in the context's scratch.
 template <typename T>
-int fused_step(mpdata_ctx* c, int iters, const T* x, T* xout, const T* u1, const T* u2,
-               const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
-               hipStream_t s) {
+int fused_step(mpdata_ctx* c, int variant, int iters, int nonos, const T* x, T* xout,
+               const T* u1, const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
+               const T* bcy, hipStream_t s) {
+    if (variant == MPDATA_STANDARD) {
+        const hipError_t e = FusedStd<T>::step(c->g, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s);
+        if (e != hipSuccess) return hip_fail(e, "fused standard step");
+        return MPDATA_OK;
+    }
     FusedScratch<T> sc;
     if (iters == 3) {
         Scratch& S = scr<T>(c);
@@ -312,7 +318,7 @@ int step_impl(mpdata_ctx* c, int variant, int iters, int nonos, T* x, const T* u
         return step_stages<T>(c, variant, iters, nonos, x, u1, u2, u3, rhr, h, bcx, bcy, s);
     if ((rc = ensure(c, &S.xnext, SCAL, sizeof(T)))) return rc;
     T* xn = (T*)S.xnext;
-    if ((rc = fused_step<T>(c, iters, x, xn, u1, u2, u3, rhr, h, bcx, bcy, s))) return rc;
+    if ((rc = fused_step<T>(c, variant, iters, nonos, x, xn, u1, u2, u3, rhr, h, bcx, bcy, s))) return rc;
     HIPCHK(hipMemcpyAsync(x, xn, (size_t)len_of<SCAL>(c->g) * sizeof(T), hipMemcpyDeviceToDevice, s));
     return MPDATA_OK;
 }
@@ -332,7 +338,7 @@ int step_oop_impl(mpdata_ctx* c, int variant, int iters, int nonos, const T* x_i
     if (a < b + bytes && b < a + bytes) return fail(MPDATA_EINVAL, "x_in and x_out overlap");
     if ((rc = use_fused<T>(c, variant, iters, nonos, &fused))) return rc;
     if (fused) {
-        return fused_step<T>(c, iters, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, s);
+        return fused_step<T>(c, variant, iters, nonos, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, s);
     }
     HIPCHK(hipMemcpyAsync(x_out, x_in, bytes, hipMemcpyDeviceToDevice, s));
     return step_stages<T>(c, variant, iters, nonos, x_out, u1, u2, u3, rhr, h, bcx, bcy, s);
@@ -360,8 +366,8 @@ int timeloop_impl(mpdata_ctx* c, int variant, int iters, int nonos, int nt, T* x
     if ((rc = ensure(c, &S.xnext, SCAL, sizeof(T)))) return rc;
     T* buf[2] = {x, (T*)S.xnext};
     for (int it = 0; it < nt; it++)
-        if ((rc = fused_step<T>(c, iters, buf[it & 1], buf[(it + 1) & 1], u1, u2, u3, rhr, h, bcx,
-                                bcy, s)))
+        if ((rc = fused_step<T>(c, variant, iters, nonos, buf[it & 1], buf[(it + 1) & 1], u1, u2,
+                                u3, rhr, h, bcx, bcy, s)))
             return rc;
     if (nt & 1)
         HIPCHK(hipMemcpyAsync(x, buf[1], (size_t)len_of<SCAL>(c->g) * sizeof(T),

@@ -8,8 +8,14 @@ namespace mpdata {
 
 template <typename T>
 bool Fused<T>::supports(const Geo& g, int variant, int iters, int nonos) {
-    if (variant != 1 || (iters != 2 && iters != 3) || nonos != 1) return false;
-    if (g.px || g.py || g.pz) return false;
+    if (variant == 2) {
+        // the standard kernel (mpdata_fused_std.hip): 2 iterations, limiter
+        // on or off, any boundaries
+        if (iters != 2 || (nonos != 0 && nonos != 1)) return false;
+    } else {
+        if (variant != 1 || (iters != 2 && iters != 3) || nonos != 1) return false;
+        if (g.px || g.py || g.pz) return false;
+    }
     // 32-bit element offsets inside the kernel, and 32-bit byte offsets
     // inside one plane of every array
     const long long biggest = len_of<FACEX>(g) > len_of<FACEY>(g) ? len_of<FACEX>(g) : len_of<FACEY>(g);
@@ -32,7 +38,8 @@ hipError_t FusedDens<T>::step(const Geo& g, const T* x, T* xout, const T* u1, co
 
 template <typename T>
 const char* Fused<T>::scope() {
-    return "the gauge variant with 2 or 3 iterations, the limiter on and open boundaries";
+    return "the gauge variant with 2 or 3 iterations, the limiter on and open boundaries, "
+           "and the standard variant with 2 iterations";
 }
 
 template <typename T>

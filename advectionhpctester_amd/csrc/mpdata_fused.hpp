@@ -22,8 +22,10 @@ struct PassIO {
 
 template <typename T>
 struct Fused {
-    // True when the fused kernel implements this configuration (gauge,
-    // limiter on, open walls, 2 or 3 iterations).
+    // True when a fused kernel implements this configuration: the gauge
+    // variant with the limiter on, open walls and 2 or 3 iterations (this
+    // struct's step), or the standard variant with 2 iterations, the limiter
+    // on or off and any boundaries (FusedStd).
     static bool supports(const Geo& g, int variant, int iters, int nonos);
     // Human-readable scope of the fused kernel, for error messages.
     static const char* scope();
@@ -47,6 +49,18 @@ struct FusedFast {
     static hipError_t step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
                            const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
                            hipStream_t s, int iters = 2, const FusedScratch<T>* sc = nullptr);
+};
+
+// The fused step of the standard antidiffusive variant, 2 iterations, limiter
+// on (nonos = 1) or off (0), open or periodic in each direction
+// (mpdata_fused_std.hip).  Exactly rounded, bit-identical to the per-stage
+// kernels; the same layout and halo results as Fused (periodic directions'
+// halo cells keep their x values).
+template <typename T>
+struct FusedStd {
+    static hipError_t step(const Geo& g, int nonos, const T* x, T* xout, const T* u1,
+                           const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
+                           const T* bcy, hipStream_t s);
 };
 
 // Extra arrays of the density step (the reference's mpdata3d_rho_gauge_gpubc,

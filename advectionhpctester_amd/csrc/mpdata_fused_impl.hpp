@@ -757,7 +757,8 @@ k_fused_gauge2(Geo g, Off o, int ntx, int nty, int nchunk, int clen,
 }
 
 // Halo cells of x_out that the step does not define keep their x_in values:
-// the z-halo planes and the x/y halo cells outside the edge copies.  Cell q
+// the z-halo planes and the x/y halo cells outside the edge copies (all x/y
+// halo cells of a periodic direction, which the step leaves alone).  Cell q
 // of the three halo slabs -> its offset, or -1 for an edge copy (written by
 // the step itself).
 __device__ __forceinline__ long long untouched_halo_cell(const Geo& g, long long q,
@@ -779,7 +780,7 @@ __device__ __forceinline__ long long untouched_halo_cell(const Geo& g, long long
         int row = (int)(tt % (2 * ih));
         k = (int)(tt / (2 * ih)) + 1;
         j = row < ih ? 1 - ih + row : g.mp + 1 + (row - ih);
-        copy = !((j == 0 || j == g.mp + 1) && i >= 1 && i <= g.np);
+        copy = g.py || !((j == 0 || j == g.mp + 1) && i >= 1 && i <= g.np);
     } else {
         long long r = q - nz_slab - ny_slab;
         int col = (int)(r % (2 * ih));
@@ -787,7 +788,7 @@ __device__ __forceinline__ long long untouched_halo_cell(const Geo& g, long long
         j = (int)(tt % g.mp) + 1;
         k = (int)(tt / g.mp) + 1;
         i = col < ih ? 1 - ih + col : g.np + 1 + (col - ih);
-        copy = !(i == 0 || i == g.np + 1);
+        copy = g.px || !(i == 0 || i == g.np + 1);
     }
     return copy ? off_raw<SCAL>(g, i, j, k) : -1;
 }
