@@ -17,6 +17,7 @@
 
 #include "../../include/mpdata_hip.h"
 #include "mpdata_ctx.hpp"
+#include "mpdata_fused.hpp"
 #include "mpdata_stages.hpp"
 #include "mpdata_status.hpp"
 
@@ -696,10 +697,24 @@ int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x
         vl[d] = (R*)t.vl[d];
         f[d] = (R*)S.f[d];
     }
-    // the stage sequence of the standard non-oscillatory step
     k_prep_x<<<dim3((g.np + 2 * g.ih + 63) / 64, (g.mp + 2 * g.ih + 3) / 4, g.lp + 2 * g.ih),
                dim3(64, 4, 1), 0, s>>>(g, x_in, bcx, bcy, xb, x0);
     HIPCHK(hipGetLastError());
+    if (c->path != MPDATA_PATH_STAGES) {
+        // the fused standard step, writing the tape as it goes
+        StdTape<R> tp;
+        tp.psi = psi;
+        tp.cp = cp;
+        tp.cn = cn;
+        for (int d = 0; d < 3; d++) {
+            tp.v[d] = v[d];
+            tp.w[d] = fns[d];
+            tp.vl[d] = vl[d];
+        }
+        HIPCHK(FusedStd<R>::record(g, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, tp, s));
+        return MPDATA_OK;
+    }
+    // the stage sequence of the standard non-oscillatory step
     HIPCHK(St::upwind(g, u1, u2, u3, xb, psi, rhr, h, s));
     HIPCHK(St::halo_xy(g, psi, St::MIRROR, 1, s));
     HIPCHK(St::pseudo_vel(g, MPDATA_STANDARD, u1, u2, u3, psi, h, v[0], v[1], v[2], fns[0],

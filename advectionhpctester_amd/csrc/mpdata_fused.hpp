@@ -57,10 +57,32 @@ struct FusedFast {
 // kernels; the same layout and halo results as Fused (periodic directions'
 // halo cells keep their x values).
 template <typename T>
+struct StdTape;
+
+template <typename T>
 struct FusedStd {
     static hipError_t step(const Geo& g, int nonos, const T* x, T* xout, const T* u1,
                            const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
                            const T* bcy, hipStream_t s);
+    // The limited step with open walls and ih = 1 that also records the
+    // adjoint's tape (StdTape); xout as step.
+    static hipError_t record(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                             const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+                             const StdTape<T>& tp, hipStream_t s);
+};
+
+// Intermediates of a standard limited step that the adjoint step reads: the
+// upwind result xant (psi) with its mirrored x/y halo ring, the
+// pseudo-velocities V, their donor fluxes w (the limiter's inputs) and the
+// limited velocities vl on the x-, y- and z-faces, and cp, cn.
+template <typename T>
+struct StdTape {
+    T* psi = nullptr;
+    T* v[3] = {nullptr, nullptr, nullptr};
+    T* w[3] = {nullptr, nullptr, nullptr};
+    T* vl[3] = {nullptr, nullptr, nullptr};
+    T* cp = nullptr;
+    T* cn = nullptr;
 };
 
 // Extra arrays of the density step (the reference's mpdata3d_rho_gauge_gpubc,

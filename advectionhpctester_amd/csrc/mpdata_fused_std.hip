@@ -77,7 +77,13 @@ __device__ __forceinline__ T slim(T v, T a1, T a2, T xm, T xc) {
 
 // One work item (see fused_item in mpdata_fused_impl.hpp for the k-march).
 // PER: any of g.px, g.py, g.pz may be set (otherwise they are taken as 0).
-template <typename T, int RX, int RY, bool NONOS, bool PER>
+// TAPE (limiter on, open boundaries, ih = 1): the step also writes the
+// intermediates the adjoint step reads to tp, as the per-stage recording
+// leaves them: xant with its mirrored x/y halo ring, V, the donor fluxes of
+// V and the limited velocities on every face of the domain (zero on the x/y
+// walls, mirrored on the lids), cp and cn.  Each cell and face is written by
+// the one position that owns it.
+template <typename T, int RX, int RY, bool NONOS, bool PER, bool TAPE = false>
 __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int ntx, int tile,
                                                int k0, int k1, const T* __restrict__ x,
                                                const T* __restrict__ u1, const T* __restrict__ u2,
@@ -86,7 +92,9 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
                                                const T* __restrict__ h,
                                                const T* __restrict__ bcx,
                                                const T* __restrict__ bcy, T* __restrict__ xo,
-                                               StdLds<T, RX, RY>& L) {
+                                               StdLds<T, RX, RY>& L,
+                                               const StdTape<T>* tp = nullptr) {
+    static_assert(!TAPE || (NONOS && !PER), "the tape covers the limited step with open walls");
     constexpr int NT = RX * RY, TX = RX - 4, TY = RY - 4;
     constexpr int SW = RX + 2, SH = RY + 2, NRING = 2 * SW + 2 * (SH - 2);
     static_assert((64 % RX == 0 || RX % 64 == 0) && NT % 64 == 0, "row layout of the waves");
@@ -162,6 +170,23 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
         sob = uint32_t(o.f2(ai, jb, 1 - o.ih)) * uint32_t(sizeof(T));
         fjmb = uint32_t(o.f1(ib, jm, 1 - o.ih)) * uint32_t(sizeof(T));
     }
+    // tape ownership: x/y halo cells of xant and the wall faces n+1 belong to
+    // the tile whose output range touches them
+    bool own_fx = false, own_fy = false, own_s = false;
+    uint32_t tso = 0, tfo = 0;
+    if constexpr (TAPE) {
+        const bool ci = li >= 0 && li < TX && gi >= 1 && gi <= np;
+        const bool cj = lj >= 0 && lj < TY && gj >= 1 && gj <= mp;
+        const bool hi = (gi == 0 && li == -1) || (gi == np + 1 && li >= 0 && li <= TX);
+        const bool hj = (gj == 0 && lj == -1) || (gj == mp + 1 && lj >= 0 && lj <= TY);
+        own_fx = (ci || (gi == np + 1 && li >= 0 && li <= TX)) && cj;
+        own_fy = (cj || (gj == mp + 1 && lj >= 0 && lj <= TY)) && ci;
+        own_s = (ci || hi) && (cj || hj);
+        if (own_s || own_fx || own_fy) {
+            tso = uint32_t(o.s(gi, gj, 1 - o.ih)) * uint32_t(sizeof(T));
+            tfo = uint32_t(o.f1(gi, gj, 1 - o.ih)) * uint32_t(sizeof(T));
+        }
+    }
     const int ihm = o.ih - 1;
     auto pl = [&](auto* a, int stride, int k) { return a + (long long)stride * (k + ihm); };
     // loads of the source column's faces i+1 / j+1 and of column j-1 on plane k
@@ -199,6 +224,7 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
     T v3q[2] = {T(0), T(0)}, w3q[2] = {T(0), T(0)};
     T xmx[2] = {T(0), T(0)}, xmn[2] = {T(0), T(0)};
     T f3c = T(0);  // flux through the upper z-face of the last output plane
+    T vl3c = T(0); // its limited velocity (TAPE)
 
     sx[0][pj + 1][pi + 1] = valid ? xq[1] : ld(own, Ks);
     if (t < NRING) sx[0][rsj][rsi] = ld(ring, Ks);
@@ -302,26 +328,32 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
         //      V of Q from the previous step, cp/cn of Q and Q-1; formed
         //      before B so that V of Q is dead before V of P is formed ------
         if (Dq && do_f) {
-            T F1 = T(0), F2 = T(0);
+            T F1 = T(0), F2 = T(0), vl1 = T(0), vl2 = T(0);
             if (!wall1) {
-                T vl = v1Q;
+                vl1 = v1Q;
                 if constexpr (NONOS)
-                    vl = slim(vl, min3(T(1), CP(2, li, lj), CN(2, li - 1, lj)),
-                              min3(T(1), CP(2, li - 1, lj), CN(2, li, lj)), XA(2, li - 1, lj),
-                              XA(2, li, lj));
-                F1 = donor(RC(2, li - 1, lj), RC(2, li, lj), vl);
+                    vl1 = slim(vl1, min3(T(1), CP(2, li, lj), CN(2, li - 1, lj)),
+                               min3(T(1), CP(2, li - 1, lj), CN(2, li, lj)), XA(2, li - 1, lj),
+                               XA(2, li, lj));
+                F1 = donor(RC(2, li - 1, lj), RC(2, li, lj), vl1);
             }
             if (!wall2) {
-                T vl = v2Q;
+                vl2 = v2Q;
                 if constexpr (NONOS)
-                    vl = slim(vl, min3(T(1), CP(2, li, lj), CN(2, li, lj - 1)),
-                              min3(T(1), CP(2, li, lj - 1), CN(2, li, lj)), XA(2, li, lj - 1),
-                              XA(2, li, lj));
-                F2 = donor(RC(2, li, lj - 1), RC(2, li, lj), vl);
+                    vl2 = slim(vl2, min3(T(1), CP(2, li, lj), CN(2, li, lj - 1)),
+                               min3(T(1), CP(2, li, lj - 1), CN(2, li, lj)), XA(2, li, lj - 1),
+                               XA(2, li, lj));
+                F2 = donor(RC(2, li, lj - 1), RC(2, li, lj), vl2);
             }
             L.sf1[pj][pi] = F1;
             L.sf2[pj][pi] = F2;
+            if constexpr (TAPE) {
+                if (own_fx) sto(pl(tp->vl[0], FXY, Q), tfo, vl1);
+                if (own_fy) sto(pl(tp->vl[1], YXY, Q), tso, vl2);
+            }
         }
+        if constexpr (TAPE)
+            if (Dq && own_s) sto(pl(tp->psi, SXY, Q), tso, XA(2, li, lj));
         // ---- B: V3 on face K, V1/V2 on plane P (antidiff3d-standard-
         //      gpubc.f90:72-202), their donor fluxes for the limiter -------
         T v3n = T(0), w3n = T(0);
@@ -414,6 +446,18 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
             v1P = val1;
             v2P = val2;
             if constexpr (NONOS) L.sw2[pj][pi] = w2;
+            if constexpr (TAPE) {
+                if (P >= k0 && P <= k1) {
+                    if (own_fx) {
+                        sto(pl(tp->v[0], FXY, P), tfo, val1);
+                        sto(pl(tp->w[0], FXY, P), tfo, w1);
+                    }
+                    if (own_fy) {
+                        sto(pl(tp->v[1], YXY, P), tso, val2);
+                        sto(pl(tp->w[1], YXY, P), tso, w2);
+                    }
+                }
+            }
         }
         // the oldest slot of each queue becomes the newest of step K+1
         {
@@ -471,16 +515,21 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
         }
         // limited flux through face Q+1 of the own column (cp/cn of Q from
         // the previous step, of Q+1 from this one)
+        T vlu = T(0);  // the limited velocity through that face (TAPE)
         auto flux_up = [&]() -> T {
             T vl = v3q2;
             if constexpr (NONOS)
                 vl = slim(vl, min3(T(1), CP(1, li, lj), CN(2, li, lj)),
                           min3(T(1), CP(2, li, lj), CN(1, li, lj)), XA(2, li, lj), XA(1, li, lj));
+            vlu = vl;
             return donor(RC(2, li, lj), RC(1, li, lj), vl);
         };
         // the lower z-face flux of the first output plane of the range is
         // the upper face of plane k0-1, formed on the step before
-        if (Q == k0 - 1 && (pz || Q >= 1) && do_d) f3c = flux_up();
+        if (Q == k0 - 1 && (pz || Q >= 1) && do_d) {
+            f3c = flux_up();
+            vl3c = vlu;
+        }
         // ---- D: the new x on plane Q (antidiff3d-standard-gpubc.f90:431-459)
         if (D_on) {
             const bool gnd = !pz && Q == 1, sky = !pz && Q == lp;
@@ -498,6 +547,31 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
                 sum = sum + F3p - f3c;
             }
             f3c = F3p;
+            if constexpr (TAPE) {
+                // z-faces: the lower face Q of the cell (face 1 = -face 2 on
+                // the ground), and face lp+1 = -face lp on the sky plane;
+                // cp and cn of the cell
+                T* const tv = pl(tp->v[2], SXY, Q);
+                T* const tw = pl(tp->w[2], SXY, Q);
+                T* const tl = pl(tp->vl[2], SXY, Q);
+                if (gnd) {
+                    sto(tv, so, -v3q2);
+                    sto(tw, so, -w3q2);
+                    sto(tl, so, -vlu);
+                } else {
+                    sto(tv, so, v3q1);
+                    sto(tw, so, w3q1);
+                    sto(tl, so, vl3c);
+                }
+                if (sky) {
+                    sto(tv + SXY, so, -v3q1);
+                    sto(tw + SXY, so, -w3q1);
+                    sto(tl + SXY, so, -vl3c);
+                }
+                sto(pl(tp->cp, SXY, Q), so, CP(2, li, lj));
+                sto(pl(tp->cn, SXY, Q), so, CN(2, li, lj));
+                vl3c = vlu;
+            }
             const T hinv = T(1) / hQ;
             T* const xoq = pl(xo, SXY, Q);
             sto(xoq, so, rQ * (RC(2, li, lj) - sum * hinv));
@@ -535,12 +609,12 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
     }
 }
 
-template <typename T, int RX, int RY, int MINB, bool NONOS, bool PER>
+template <typename T, int RX, int RY, int MINB, bool NONOS, bool PER, bool TAPE = false>
 __global__ void __launch_bounds__(RX * RY, MINB)
 k_fused_std(Geo g, Off o, int ntx, int nty, int nchunk, int clen, const T* __restrict__ x,
             const T* __restrict__ u1, const T* __restrict__ u2, const T* __restrict__ u3,
             const T* __restrict__ rhr, const T* __restrict__ h, const T* __restrict__ bcx,
-            const T* __restrict__ bcy, T* __restrict__ xo) {
+            const T* __restrict__ bcy, T* __restrict__ xo, StdTape<T> tp) {
     __shared__ StdLds<T, RX, RY> L;
     // work-item order of k_fused_gauge2: contiguous tile runs per XCD
     const int ntile = ntx * nty;
@@ -556,15 +630,15 @@ k_fused_std(Geo g, Off o, int ntx, int nty, int nchunk, int clen, const T* __res
         const int tile = xcd * tiles_per_xcd + tix;
         if (tile >= ntile) continue;
         const int k0 = 1 + chunk * clen, k1 = min(g.lp, k0 + clen - 1);
-        fused_std_item<T, RX, RY, NONOS, PER>(g, o, ntx, tile, k0, k1, x, u1, u2, u3, rhr, h, bcx,
-                                              bcy, xo, L);
+        fused_std_item<T, RX, RY, NONOS, PER, TAPE>(g, o, ntx, tile, k0, k1, x, u1, u2, u3, rhr, h,
+                                                    bcx, bcy, xo, L, &tp);
     }
 }
 
-template <typename T, int RX, int RY, int MINB, bool NONOS, bool PER>
+template <typename T, int RX, int RY, int MINB, bool NONOS, bool PER, bool TAPE = false>
 hipError_t launch_std(const Geo& g, const Off& o, int clen, const T* x, T* xout, const T* u1,
                       const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
-                      const T* bcy, hipStream_t s) {
+                      const T* bcy, hipStream_t s, const StdTape<T>& tp = StdTape<T>{}) {
     const int ntx = (g.np + RX - 5) / (RX - 4), nty = (g.mp + RY - 5) / (RY - 4);
     const int nchunk = (g.lp + clen - 1) / clen;
     const int nitem = ntx * nty * nchunk;
@@ -573,12 +647,12 @@ hipError_t launch_std(const Geo& g, const Off& o, int clen, const T* x, T* xout,
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_fused_std<T, RX, RY, MINB, NONOS, PER>, RX * RY, 0);
+            &per_cu, k_fused_std<T, RX, RY, MINB, NONOS, PER, TAPE>, RX * RY, 0);
     if (e != hipSuccess) return e;
     const int grid = persistent_grid(nitem, (long long)(ntx * nty + NXCD - 1) / NXCD * nchunk, ncu,
                                      per_cu);
-    k_fused_std<T, RX, RY, MINB, NONOS, PER><<<grid, RX * RY, 0, s>>>(
-        g, o, ntx, nty, nchunk, clen, x, u1, u2, u3, rhr, h, bcx, bcy, xout);
+    k_fused_std<T, RX, RY, MINB, NONOS, PER, TAPE><<<grid, RX * RY, 0, s>>>(
+        g, o, ntx, nty, nchunk, clen, x, u1, u2, u3, rhr, h, bcx, bcy, xout, tp);
     return hipGetLastError();
 }
 
@@ -607,6 +681,21 @@ hipError_t FusedStd<T>::step(const Geo& g, int nonos, const T* x, T* xout, const
     hipError_t e = cfg == 0
                        ? launch_std_any<T, 32, 32, 4>(g, o, cl, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s)
                        : launch_std_any<T, 32, 16, 4>(g, o, cl, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s);
+    if (e != hipSuccess) return e;
+    k_copy_untouched_halo<T><<<256, 256, 0, s>>>(g, x, xout);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t FusedStd<T>::record(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                               const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+                               const StdTape<T>& tp, hipStream_t s) {
+    if (g.px || g.py || g.pz || g.ih != 1) return hipErrorInvalidValue;
+    const Off o = make_off(g);
+    int cfg = 0, cl = 64;
+    choose_tiling<T>(g, cfg, cl);
+    hipError_t e = launch_std<T, 32, 16, 2, true, false, true>(g, o, cl, x, xout, u1, u2, u3, rhr,
+                                                               h, bcx, bcy, s, tp);
     if (e != hipSuccess) return e;
     k_copy_untouched_halo<T><<<256, 256, 0, s>>>(g, x, xout);
     return hipGetLastError();

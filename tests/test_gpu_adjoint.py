@@ -132,6 +132,38 @@ def test_adjoint_matches_reference_gradient(mpd, dims, seed, positive):
         assert err < 1e-12, name
 
 
+@pytest.mark.parametrize("dims,seed,positive,chunk", [((20, 16, 12, 1), 31, True, None),
+                                                      ((13, 18, 7, 1), 32, False, None),
+                                                      ((61, 37, 11, 1), 33, False, "3"),
+                                                      ((61, 37, 11, 1), 34, True, "7")])
+def test_fused_tape_equals_stage_tape(mpd, dims, seed, positive, chunk, monkeypatch):
+    """The fused recording (the fused standard step writing the tape, the
+    default path) and the per-stage recording give the same step and the
+    same gradients: every tape value the adjoint kernels read is the same.
+    The adjoint sums scattered contributions with atomics, whose order may
+    differ between runs, so the gradients agree to rounding."""
+    if chunk is not None:
+        monkeypatch.setenv("MPDATA_FUSED_CHUNK", chunk)
+    nsteps = 3
+    c = _case(dims, seed, positive)
+    rng = np.random.default_rng(seed)
+    xbar = _zero_halo(rng.uniform(-1, 1, c["x"].shape))
+    d = {k: to_dev(v) for k, v in c.items()}
+    res = {}
+    for name, path in (("stages", mpd.PATH_STAGES), ("fused", mpd.PATH_AUTO)):
+        with mpd.Domain(*dims) as dom:
+            dom.set_path(path)
+            print(f"tape {name} dims={dims} chunk={chunk}", flush=True)
+            res[name] = _adjoint_chain(mpd, dom, d, nsteps, xbar)
+    (gs, xs), (gf, xf) = res["stages"], res["fused"]
+    assert bits_equal(xs[I], xf[I])
+    for name, a_, b_ in zip(("x", "u1", "u2", "u3"), gs, gf):
+        sl = I if name == "x" else (slice(None),) * 3
+        err = np.abs(a_[sl] - b_[sl]).max() / np.abs(a_[sl]).max()
+        print(f"fused vs stage tape, gradient {name}: max rel {err:.2e}", flush=True)
+        assert err < 1e-13, name
+
+
 @pytest.mark.parametrize("nsteps", [1, 5])
 def test_dot_product(mpd, nsteps):
     """<TL(dx, du), ybar> = <dx, xbar> + <du, ubar>."""
