@@ -2,11 +2,14 @@
 hand-written adjoint kernels of libmpdata_hip (standard non-oscillatory
 scheme, 2 iterations, open walls, FP64).
 
-Forward runs ``nsteps`` plain steps and keeps the field every ``ckpt`` steps.
-Backward walks the checkpoint segments from the last: it recomputes the
-segment's steps from its checkpoint, recording each step's intermediates on
-the library's tape (one slot per step of a segment), then runs the adjoint
-steps in reverse.  Memory: ``nsteps/ckpt`` fields plus ``ckpt`` tape slots.
+Forward runs ``nsteps`` steps and keeps the field every ``ckpt`` steps; the
+steps of the last segment run as recorded steps, which keep their
+intermediates on the library's tape (one slot per step of a segment).
+Backward walks the checkpoint segments from the last: the last one's tape is
+already there (unless another recording on the domain replaced it); every
+other segment is recomputed from its checkpoint with recording; then the
+adjoint steps run in reverse.  Memory: ``nsteps/ckpt`` fields plus ``ckpt``
+tape slots.
 
 The velocity of step n is ``factors[n] * (u1, u2, u3)`` (all ones by
 default), so a time-dependent flow of fixed shape, such as the reversing
@@ -34,11 +37,28 @@ class MpdataSteps(torch.autograd.Function):
             raise ValueError("one velocity factor per step")
         ckpts = []
         x = x0.detach().clone()
+        # the last segment's steps are recorded, so that the backward starts
+        # with its tape
+        taped = any(ctx.needs_input_grad[:4])
+        last0 = ((nsteps - 1) // ckpt) * ckpt if nsteps > 0 else 0
+        if taped and nsteps > 0:
+            dom.adj_reserve(min(ckpt, nsteps))
+        spare = None
+        last_vel = []
         for n in range(nsteps):
             if n % ckpt == 0:
                 ckpts.append(x.clone())
-            dom.step(x, _scaled(u1, factors[n]), _scaled(u2, factors[n]), _scaled(u3, factors[n]),
-                     rhr, h, bcx, bcy, variant=mpdata.STANDARD, iters=2, nonos=1)
+            un = [_scaled(u, factors[n]) for u in (u1, u2, u3)]
+            if taped and n >= last0:
+                if spare is None:
+                    spare = torch.empty_like(x)
+                dom.adj_record(n - last0, x, spare, *un, rhr, h, bcx, bcy)
+                x, spare = spare, x
+                last_vel.append(un)
+            else:
+                dom.step(x, *un, rhr, h, bcx, bcy, variant=mpdata.STANDARD, iters=2, nonos=1)
+        ctx.tape_epoch = dom.tape_epoch if taped else None
+        ctx.last_vel = last_vel
         ctx.dom, ctx.nsteps, ctx.ckpt, ctx.factors = dom, nsteps, ckpt, factors
         ctx.ckpts = ckpts
         ctx.fields = (u1.detach(), u2.detach(), u3.detach(), rhr, h, bcx, bcy)
@@ -53,21 +73,27 @@ class MpdataSteps(torch.autograd.Function):
         mpdata.interior(xbar).copy_(mpdata.interior(grad))
         ub = [torch.zeros_like(u) for u in (u1, u2, u3)]
         bufs = [torch.zeros_like(grad), torch.zeros_like(grad)]
-        for seg in reversed(range(len(ctx.ckpts))):
+        nseg = len(ctx.ckpts)
+        for seg in reversed(range(nseg)):
             s0 = seg * ckpt
             s1 = min(nsteps, s0 + ckpt)
             x = ctx.ckpts[seg]
-            vel = []
-            for n in range(s0, s1):
-                un = [_scaled(u, factors[n]) for u in (u1, u2, u3)]
-                vel.append(un)
-                out = bufs[(n - s0) & 1]
-                dom.adj_record(n - s0, x, out, *un, rhr, h, bcx, bcy)
-                x = out
+            # the forward recorded the last segment (and kept its velocities);
+            # its tape is intact if nothing has recorded on this domain since
+            if (seg == nseg - 1 and ctx.tape_epoch is not None
+                    and dom.tape_epoch == ctx.tape_epoch):
+                vel = ctx.last_vel
+            else:
+                vel = [[_scaled(u, factors[n]) for u in (u1, u2, u3)] for n in range(s0, s1)]
+                for n in range(s0, s1):
+                    out = bufs[(n - s0) & 1]
+                    dom.adj_record(n - s0, x, out, *vel[n - s0], rhr, h, bcx, bcy)
+                    x = out
             for n in reversed(range(s0, s1)):
                 # u_n = f_n u: the gradient of u collects f_n times that of u_n
                 dom.adj_step(n - s0, *vel[n - s0], rhr, h, xbar, *ub, ufac=factors[n])
         ctx.ckpts = None
+        ctx.last_vel = None
         return (xbar, ub[0], ub[1], ub[2]) + (None,) * 8
 
 

@@ -5,7 +5,7 @@
 // reverse order:
 //   update (:431-459) -> limited donor fluxes (:372-407) -> limiter
 //   (:260-347) -> pseudo-velocities and first-order fluxes (:72-202) ->
-//   halo fills (mirror of xant, edge copy of x) -> upwind (:101-198).
+//   halo fills (mirror of xant, edge copy of x) and upwind (:101-198).
 // Each reverse kernel transposes the linearisation the tangent-linear step
 // (the stage kernels on dual numbers) computes: every branch is taken as the
 // forward value takes it.  Contributions from one face or cell to several
@@ -57,17 +57,40 @@ struct Adj {
     R uf;  // the velocity adjoint is added to ubar scaled by uf
 };
 
-// Zeroes up to 8 arrays in one launch (blockIdx.y = array).
+// The adjoint arrays at the start of a reverse step, in one launch: the
+// arrays that receive scattered contributions zeroed (blockIdx.y < nz), and
+// the reverse of the update out = rhr*(psi/rhr - sum*hinv)
+// (antidiff3d-standard-gpubc.f90:431-459) in the last row of blocks:
+// psi = w/rhr (zero on the halo) and sum = -(w*hinv) with w = rhr*xbar.
 struct ZeroList {
     R* p[8];
     long long n[8];
+    int nz;
 };
-__global__ void k_zero(ZeroList z) {
-    R* __restrict__ a = z.p[blockIdx.y];
-    const long long n = z.n[blockIdx.y];
-    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n;
-         q += (long long)gridDim.x * blockDim.x)
-        a[q] = R(0);
+__global__ void k_adj_init(Geo g, ZeroList z, const R* __restrict__ xbar,
+                           const R* __restrict__ rhr, const R* __restrict__ h, Adj a) {
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    if ((int)blockIdx.y < z.nz) {
+        R* __restrict__ p = z.p[blockIdx.y];
+        const long long n = z.n[blockIdx.y];
+        for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += stride)
+            p[q] = R(0);
+        return;
+    }
+    const int ih = g.ih;
+    const long long nx = g.np + 2 * ih, ny = g.mp + 2 * ih, n = len_of<SCAL>(g);
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += stride) {
+        const int i = (int)(q % nx) + 1 - ih, j = (int)((q / nx) % ny) + 1 - ih,
+                  k = (int)(q / (nx * ny)) + 1 - ih;
+        if (i >= 1 && i <= g.np && j >= 1 && j <= g.mp && k >= 1 && k <= g.lp) {
+            const R w = rhr[q] * xbar[q];
+            const R hinv = R(1) / h[q];
+            a.psi[q] = w / rhr[q];
+            a.sum[q] = -(w * hinv);
+        } else {
+            a.psi[q] = R(0);
+        }
+    }
 }
 
 // The two x inputs of a recorded step from x_in in one pass: xb with the
@@ -93,18 +116,6 @@ __global__ void k_prep_x(Geo g, const R* __restrict__ x, const R* __restrict__ b
     }
     xb[c] = vb;
     x0[c] = v0;
-}
-
-// ---- update: out = rhr*(psi/rhr - sum*hinv) (antidiff3d-standard-gpubc.f90:431-459)
-__global__ void __launch_bounds__(256) k_adj_update(Geo g, const R* __restrict__ xbar,
-                                                    const R* __restrict__ rhr,
-                                                    const R* __restrict__ h, Adj a) {
-    CELL_HEAD
-    (void)gnd; (void)sky;
-    const R w = S_(rhr, i, j, k) * S_(xbar, i, j, k);
-    const R hinv = R(1) / S_(h, i, j, k);
-    S_(a.psi, i, j, k) = w / S_(rhr, i, j, k);
-    S_(a.sum, i, j, k) = -(w * hinv);
 }
 
 // ---- block-local accumulation ---------------------------------------------
@@ -517,50 +528,46 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
     }
 }
 
-// ---- halo fills in reverse -------------------------------------------------
-// x direction (dir 0): the mirror of xant over the full (j,k) range and the
-// edge copy of x over the interior range; y direction (dir 1) likewise.  The
-// forward fills y first, then x, so the reverse runs dir 0, then dir 1.
-__global__ void k_adj_halo(Geo g, int dir, Adj a) {
-    const int ih = g.ih;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;   // along the wall
-    const int k = blockIdx.y + 1 - ih;                     // 1-ih .. lp+ih
-    if (dir == 0) {
-        const int j = p + 1 - ih;
-        if (j > g.mp + ih) return;
-        R* lo = SP(a.psi, 0, j, k);
-        R* hi = SP(a.psi, g.np + 1, j, k);
-        *SP(a.psi, 2, j, k) += *lo;
-        *lo = R(0);
-        *SP(a.psi, g.np - 1, j, k) += *hi;
-        *hi = R(0);
-        if (j >= 1 && j <= g.mp && k >= 1 && k <= g.lp) {
-            *SP(a.x0, 1, j, k) += *SP(a.x0, 0, j, k);
-            *SP(a.x0, g.np, j, k) += *SP(a.x0, g.np + 1, j, k);
-        }
-    } else {
-        const int i = p + 1 - ih;
-        if (i > g.np + ih) return;
-        R* lo = SP(a.psi, i, 0, k);
-        R* hi = SP(a.psi, i, g.mp + 1, k);
-        *SP(a.psi, i, 2, k) += *lo;
-        *lo = R(0);
-        *SP(a.psi, i, g.mp - 1, k) += *hi;
-        *hi = R(0);
-        if (i >= 1 && i <= g.np && k >= 1 && k <= g.lp) {
-            *SP(a.x0, i, 1, k) += *SP(a.x0, i, 0, k);
-            *SP(a.x0, i, g.mp, k) += *SP(a.x0, i, g.mp + 1, k);
-        }
-    }
-}
-
 // ---- upwind (upwind3d-gpubc.f90:101-198) and the sum of both uses of x -----
+// The reverse of the halo fills (mirror of xant, edge copy of x) is folded
+// in: psi_in and x0_in gather the halo cells' adjoints.
 // xant = rhr*(x - sum*hinv), fluxes donor(x(m), x(c), u); the x halo holds the
 // inflow values (no adjoint).  Each cell takes its low faces (and the last
 // cell of a direction its high wall face) for the velocity adjoint.
+// The adjoint of xant at interior cell (i,j,k) after the reverse halo fills
+// (the forward mirrored xant over the full range, y walls first, then x
+// walls; the reverse adds each halo cell's adjoint into its source, x walls
+// first): its own value plus the x-wall cell it mirrors to, and on the rows
+// that the y walls mirror, the y-wall cell with the x-wall cells that were
+// moved into it.
+__device__ __forceinline__ R psi_in(const Geo& g, const R* __restrict__ p, int i, int j, int k) {
+    const int np = g.np, mp = g.mp;
+    auto row = [&](int jj) {
+        R v = S_(p, i, jj, k);
+        if (i == 2) v += S_(p, 0, jj, k);
+        if (i == np - 1) v += S_(p, np + 1, jj, k);
+        return v;
+    };
+    R v = row(j);
+    if (j == 2) v += row(0);
+    if (j == mp - 1) v += row(mp + 1);
+    return v;
+}
+
+// The adjoint of the edge-copied x at interior cell (i,j,k): its own value
+// plus the wall halo cells that copy it.
+__device__ __forceinline__ R x0_in(const Geo& g, const R* __restrict__ p, int i, int j, int k) {
+    R v = S_(p, i, j, k);
+    if (i == 1) v += S_(p, 0, j, k);
+    if (i == g.np) v += S_(p, g.np + 1, j, k);
+    if (j == 1) v += S_(p, i, 0, k);
+    if (j == g.mp) v += S_(p, i, g.mp + 1, k);
+    return v;
+}
+
 __device__ __forceinline__ R sbar_of(Geo g, const Adj& a, const R* rhr, const R* h, int i, int j,
                                      int k) {
-    return -(S_(rhr, i, j, k) * S_(a.psi, i, j, k) * (R(1) / S_(h, i, j, k)));
+    return -(S_(rhr, i, j, k) * psi_in(g, a.psi, i, j, k) * (R(1) / S_(h, i, j, k)));
 }
 
 __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__ u1,
@@ -573,9 +580,9 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
                                                     R* __restrict__ ub3, Adj a) {
     CELL_HEAD
     const int np = g.np, mp = g.mp, lp = g.lp;
-    const R w = S_(rhr, i, j, k) * S_(a.psi, i, j, k);
+    const R w = S_(rhr, i, j, k) * psi_in(g, a.psi, i, j, k);
     const R sc = -(w * (R(1) / S_(h, i, j, k)));
-    R xb_c = w + S_(a.x0, i, j, k);
+    R xb_c = w + x0_in(g, a.x0, i, j, k);
     const R xc = S_(xb, i, j, k);
     // x faces i (c downstream) and i+1 (c upstream)
     {
@@ -697,12 +704,11 @@ int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x
         vl[d] = (R*)t.vl[d];
         f[d] = (R*)S.f[d];
     }
-    k_prep_x<<<dim3((g.np + 2 * g.ih + 63) / 64, (g.mp + 2 * g.ih + 3) / 4, g.lp + 2 * g.ih),
-               dim3(64, 4, 1), 0, s>>>(g, x_in, bcx, bcy, xb, x0);
-    HIPCHK(hipGetLastError());
     if (c->path != MPDATA_PATH_STAGES) {
         // the fused standard step, writing the tape as it goes
         StdTape<R> tp;
+        tp.xb = xb;
+        tp.x0 = x0;
         tp.psi = psi;
         tp.cp = cp;
         tp.cn = cn;
@@ -715,6 +721,9 @@ int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x
         return MPDATA_OK;
     }
     // the stage sequence of the standard non-oscillatory step
+    k_prep_x<<<dim3((g.np + 2 * g.ih + 63) / 64, (g.mp + 2 * g.ih + 3) / 4, g.lp + 2 * g.ih),
+               dim3(64, 4, 1), 0, s>>>(g, x_in, bcx, bcy, xb, x0);
+    HIPCHK(hipGetLastError());
     HIPCHK(St::upwind(g, u1, u2, u3, xb, psi, rhr, h, s));
     HIPCHK(St::halo_xy(g, psi, St::MIRROR, 1, s));
     HIPCHK(St::pseudo_vel(g, MPDATA_STANDARD, u1, u2, u3, psi, h, v[0], v[1], v[2], fns[0],
@@ -749,21 +758,23 @@ int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double*
           (R*)c->adj[A_F1], (R*)c->adj[A_F2], (R*)c->adj[A_F3], ufac};
     const TapeSlot& t = c->tape[slot];
     hipStream_t s = (hipStream_t)stream;
-    // arrays that receive scattered contributions start at zero; a.sum and
-    // the pseudo-velocity adjoints are written before they are read
+    // arrays that receive scattered contributions start at zero, a.psi and
+    // a.sum from the update's reverse; the pseudo-velocity adjoints are
+    // written before they are read
     {
-        const int zq[7] = {A_PSI, A_X0, A_CP, A_CN, A_F1, A_F2, A_F3};
+        const int zq[6] = {A_X0, A_CP, A_CN, A_F1, A_F2, A_F3};
         ZeroList z;
-        long long nmax = 0;
-        for (int q = 0; q < 7; q++) {
+        z.nz = 6;
+        long long nmax = len_of<SCAL>(g);
+        for (int q = 0; q < 6; q++) {
             z.p[q] = (R*)c->adj[zq[q]];
             z.n[q] = field_len(g, A_KIND[zq[q]]);
             nmax = std::max(nmax, z.n[q]);
         }
-        k_zero<<<dim3((unsigned)std::min<long long>(512, (nmax + 255) / 256), 7), 256, 0, s>>>(z);
+        k_adj_init<<<dim3((unsigned)std::min<long long>(512, (nmax + 255) / 256), 7), 256, 0, s>>>(
+            g, z, xbar, rhr, h, a);
     }
     HIPCHK(hipGetLastError());
-    k_adj_update<<<cgrid(g), cblock(), 0, s>>>(g, xbar, rhr, h, a);
     k_adj_flux<<<dim3((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, g.lp), dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.v[0], (const R*)t.v[1],
                                              (const R*)t.v[2], (const R*)t.vl[0],
                                              (const R*)t.vl[1], (const R*)t.vl[2],
@@ -776,9 +787,6 @@ int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double*
                    s>>>(g, u1, u2, u3, (const R*)t.psi, h,
                                                (const R*)t.v[0], (const R*)t.v[1],
                                                (const R*)t.v[2], ubar1, ubar2, ubar3, a);
-    const int wall_y = g.mp + 2 * g.ih, wall_x = g.np + 2 * g.ih, planes = g.lp + 2 * g.ih;
-    k_adj_halo<<<dim3((wall_y + 63) / 64, planes), 64, 0, s>>>(g, 0, a);
-    k_adj_halo<<<dim3((wall_x + 63) / 64, planes), 64, 0, s>>>(g, 1, a);
     k_adj_upwind<<<cgrid(g), cblock(), 0, s>>>(g, u1, u2, u3, (const R*)t.xb, rhr, h, xbar, ubar1,
                                                ubar2, ubar3, a);
     HIPCHK(hipGetLastError());

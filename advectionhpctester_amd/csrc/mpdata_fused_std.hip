@@ -79,10 +79,11 @@ __device__ __forceinline__ T slim(T v, T a1, T a2, T xm, T xc) {
 // PER: any of g.px, g.py, g.pz may be set (otherwise they are taken as 0).
 // TAPE (limiter on, open boundaries, ih = 1): the step also writes the
 // intermediates the adjoint step reads to tp, as the per-stage recording
-// leaves them: xant with its mirrored x/y halo ring, V, the donor fluxes of
-// V and the limited velocities on every face of the domain (zero on the x/y
-// walls, mirrored on the lids), cp and cn.  Each cell and face is written by
-// the one position that owns it.
+// leaves them: x with inflow values / edge copies on the wall halo, xant
+// with its mirrored x/y halo ring, V, the donor fluxes of V and the limited
+// velocities on every face of the domain (zero on the x/y walls, mirrored on
+// the lids), cp and cn.  Each cell and face is written by the one position
+// that owns it.
 template <typename T, int RX, int RY, bool NONOS, bool PER, bool TAPE = false>
 __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int ntx, int tile,
                                                int k0, int k1, const T* __restrict__ x,
@@ -571,6 +572,17 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
                 sto(pl(tp->cp, SXY, Q), so, CP(2, li, lj));
                 sto(pl(tp->cn, SXY, Q), so, CN(2, li, lj));
                 vl3c = vlu;
+                // x with the inflow values on the wall halo (the upwind's
+                // input) and with edge copies there (the limiter's bounds)
+                const T xc = ldo(pl(x, SXY, Q), so);
+                T* const tb = pl(tp->xb, SXY, Q);
+                T* const t0 = pl(tp->x0, SXY, Q);
+                sto(tb, so, xc);
+                sto(t0, so, xc);
+                if (gi == 1) { sto(tb - 1, so, bcx[(gj - 1) + mp * (Q - 1)]); sto(t0 - 1, so, xc); }
+                if (gi == np) { sto(tb + 1, so, bcx[(gj - 1) + mp * ((Q - 1) + lp)]); sto(t0 + 1, so, xc); }
+                if (gj == 1) { sto(tb - o.sx, so, bcy[(gi - 1) + np * (Q - 1)]); sto(t0 - o.sx, so, xc); }
+                if (gj == mp) { sto(tb + o.sx, so, bcy[(gi - 1) + np * ((Q - 1) + lp)]); sto(t0 + o.sx, so, xc); }
             }
             const T hinv = T(1) / hQ;
             T* const xoq = pl(xo, SXY, Q);

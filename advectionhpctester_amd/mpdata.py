@@ -243,7 +243,17 @@ class Domain:
                                      self._p(dx, 0, "dx"), pu1, pu2, pu3, *pd, prhr, ph, pbx, pby,
                                      self._stream()))
 
+    @property
+    def tape_epoch(self):
+        """Number of recordings made on this domain's tape: a caller that
+        recorded a step can tell whether its slots have been written since."""
+        return getattr(self, "_tape_epoch", 0)
+
+    def _tape_written(self):
+        self._tape_epoch = self.tape_epoch + 1
+
     def adj_reserve(self, nslots):
+        # slots already reserved keep their contents
         check(LIB.mpdata_adj_reserve(self._ctx, int(nslots)))
 
     def adj_record(self, slot, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy):
@@ -251,6 +261,7 @@ class Domain:
         intermediates in tape slot `slot`."""
         pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
         pbx, pby = self._bc(bcx, bcy)
+        self._tape_written()
         check(LIB.mpdata_adj_record_f64(self._ctx, int(slot), self._p(x_in, 0, "x_in"),
                                         self._p(x_out, 0, "x_out"), pu1, pu2, pu3, prhr, ph,
                                         pbx, pby, self._stream()))

@@ -347,3 +347,29 @@ def test_checkpointed_gradient_128x128x64(mpd):
         t_fb = time.perf_counter() - t0
         print(f"cost: forward {t_fwd * 1e3:.1f} ms, forward+backward {t_fb * 1e3:.1f} ms, "
               f"ratio {t_fb / t_fwd:.2f}", flush=True)
+
+
+def test_tape_replaced_between_forward_and_backward(mpd):
+    """The forward records the last checkpoint segment for the backward; a
+    recording on the same domain in between replaces that tape, and the
+    backward then recomputes the segment: same gradient either way."""
+    from advectionhpctester_amd import adjoint
+    dims = (24, 20, 10, 1)
+    c = _case(dims, 41, False)
+    d = {k: to_dev(v) for k, v in c.items()}
+    rest = (d["rhr"], d["h"], d["bcx"], d["bcy"])
+    obs = to_dev(np.random.default_rng(41).uniform(-1, 2, c["x"].shape))
+    grads = []
+    with mpd.Domain(*dims) as dom:
+        for replace in (False, True):
+            leaves = [d[k].clone().requires_grad_(True) for k in ("x", "u1", "u2", "u3")]
+            xT = adjoint.advect(*leaves, dom, *rest, 7, ckpt=3)
+            if replace:
+                print("tape replaced before the backward", flush=True)
+                dom.adj_record(0, d["x"], torch.empty_like(d["x"]), d["u1"], d["u2"], d["u3"],
+                               *rest)
+            r = mpd.interior(xT - obs)
+            (0.5 * (r * r).sum()).backward()
+            grads.append([leaf.grad.detach().clone() for leaf in leaves])
+    for g0, g1 in zip(*grads):
+        assert torch.allclose(g1, g0, rtol=1e-12, atol=1e-14 * float(g0.abs().max()))
