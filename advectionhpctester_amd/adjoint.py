@@ -26,6 +26,24 @@ def _scaled(u, f):
     return u if f == 1.0 else u * f
 
 
+def _complete_halo(x, x_prev, x0):
+    """The halo of x (ih = 1) as a plain step from x_prev leaves it, after a
+    recorded step, which defines only the interior: edge copies of x_prev (the
+    field at the step's time level n) on the x/y walls of the interior planes,
+    the other halo cells (lid planes, x/y corners) those of the input x0,
+    which no step changes."""
+    x[0].copy_(x0[0])
+    x[-1].copy_(x0[-1])
+    m = slice(1, -1)
+    x[m, m, 0].copy_(x_prev[m, m, 1])
+    x[m, m, -1].copy_(x_prev[m, m, -2])
+    x[m, 0, m].copy_(x_prev[m, 1, m])
+    x[m, -1, m].copy_(x_prev[m, -2, m])
+    for jj in (0, -1):
+        for ii in (0, -1):
+            x[m, jj, ii].copy_(x0[m, jj, ii])
+
+
 class MpdataSteps(torch.autograd.Function):
     """x_T = MPDATA^nsteps(x_0; u).  Differentiable in x0, u1, u2, u3 (the
     interiors; halo cells of x carry no gradient)."""
@@ -57,6 +75,8 @@ class MpdataSteps(torch.autograd.Function):
                 last_vel.append(un)
             else:
                 dom.step(x, *un, rhr, h, bcx, bcy, variant=mpdata.STANDARD, iters=2, nonos=1)
+        if taped and nsteps > 0:
+            _complete_halo(x, spare, x0.detach())
         ctx.tape_epoch = dom.tape_epoch if taped else None
         ctx.last_vel = last_vel
         ctx.dom, ctx.nsteps, ctx.ckpt, ctx.factors = dom, nsteps, ckpt, factors

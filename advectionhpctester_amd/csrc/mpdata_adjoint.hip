@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restr
 // adjoints a block's faces scatter land within one cell (psi) or one face (u)
 // of the block's cells, so they are summed in LDS (indices below) first and
 // the block adds its region to the global arrays once.
-constexpr int PBX = BX, PBY = BY;
+constexpr int PBX = BX, PBY = BY, PKB = 4;
 constexpr int LP_N = 3 * (PBY + 2) * (PBX + 2);   // psi cells
 constexpr int LU_N = 2 * (PBY + 1) * (PBX + 1);   // faces of one direction
 constexpr int L_U1 = LP_N, L_U2 = LP_N + LU_N, L_U3 = LP_N + 2 * LU_N, L_N = LP_N + 3 * LU_N;
@@ -420,112 +420,143 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
                                                           const R* __restrict__ v3,
                                                           R* __restrict__ ub1, R* __restrict__ ub2,
                                                           R* __restrict__ ub3, Adj a) {
-    // LDS regions: psi cells (i0-1..i0+PBX, j0-1..j0+PBY, k-1..k+1); u1 faces
-    // (i0..i0+PBX, j0-1..j0+PBY-1, k-1..k); u2 faces (i0-1..i0+PBX-1,
-    // j0..j0+PBY, k-1..k); u3 faces (i0-1..i0+PBX-1, j0-1..j0+PBY-1, k..k+1)
+    // A block marches PKB planes k.  LDS windows: psi cells (i0-1..i0+PBX,
+    // j0-1..j0+PBY) on planes k-1..k+1, in slot (plane mod 3); u1 faces
+    // (i0..i0+PBX, j0-1..j0+PBY-1) and u2 faces (i0-1..i0+PBX-1, j0..j0+PBY)
+    // on planes k-1..k, u3 faces (i0-1..i0+PBX-1, j0-1..j0+PBY-1) on k..k+1,
+    // in slot (plane mod 2).  After plane k, psi plane k-1, u1/u2 plane k-1
+    // and u3 face plane k get no further contributions from this block: they
+    // are added to the global arrays and their slots cleared for reuse.
     __shared__ R L[L_N];
     const int tid = threadIdx.y * PBX + threadIdx.x;
-    const int i0 = 1 + blockIdx.x * PBX, j0 = 1 + blockIdx.y * PBY, k = 1 + blockIdx.z;
+    const int i0 = 1 + blockIdx.x * PBX, j0 = 1 + blockIdx.y * PBY;
+    const int kb0 = 1 + blockIdx.z * PKB, kb1 = min(g.lp, kb0 + PKB - 1);
     for (int q = tid; q < L_N; q += PBX * PBY) L[q] = R(0);
     __syncthreads();
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     const bool valid = i <= g.np && j <= g.mp;
-    const bool gnd = k == 1, sky = k == g.lp;
-    const bool zterms = !(gnd || sky);
     const R ep = Eps<R>::v;
-#define LPS(ii, jj, kk) ((((kk) - k + 1) * (PBY + 2) + ((jj) - j0 + 1)) * (PBX + 2) + ((ii) - i0 + 1))
-#define LU1(ii, jj, kk) (L_U1 + (((kk) - k + 1) * (PBY + 1) + ((jj) - j0 + 1)) * (PBX + 1) + ((ii) - i0))
-#define LU2(ii, jj, kk) (L_U2 + (((kk) - k + 1) * (PBY + 1) + ((jj) - j0)) * (PBX + 1) + ((ii) - i0 + 1))
-#define LU3(ii, jj, kk) (L_U3 + (((kk) - k) * (PBY + 1) + ((jj) - j0 + 1)) * (PBX + 1) + ((ii) - i0 + 1))
-#define AXV(ii, jj, kk) S_(psi, ii, jj, kk)
-#define CORR(c, F1, F2, F3, F4, BF1, BF2, BF3, BF4, I0, J0, K0, I1, J1, K1, I2, J2, K2, I3, J3, K3) \
-    {                                                                                          \
-        c.bf[0] = BF1; c.bf[1] = BF2; c.bf[2] = BF3; c.bf[3] = BF4;                            \
-        c.b = F1 + F2 + F3 + F4;                                                               \
-        const R v0_ = AXV(I0, J0, K0), v1_ = AXV(I1, J1, K1);                                  \
-        const R v2_ = AXV(I2, J2, K2), v3_ = AXV(I3, J3, K3);                                  \
-        c.sg[0] = sgn(v0_); c.sg[1] = sgn(v1_); c.sg[2] = sgn(v2_); c.sg[3] = sgn(v3_);        \
-        c.cb[0] = LPS(I0, J0, K0); c.cb[1] = LPS(I1, J1, K1);                                  \
-        c.cb[2] = LPS(I2, J2, K2); c.cb[3] = LPS(I3, J3, K3);                                  \
-        c.y1 = (fabs(v0_) + fabs(v1_)) - (fabs(v2_) + fabs(v3_));                              \
-        c.y2 = fabs(v0_) + fabs(v1_) + fabs(v2_) + fabs(v3_) + ep;                             \
-        c.iy2 = R(1) / c.y2;                                                                   \
-    }
-    if (valid && i > 1) {
-        const R hm = R(1) / (R(0.5) * (S_(h, i - 1, j, k) + S_(h, i, j, k)));
-        Corr c0, c1;
-        CORR(c0, Y_(u2, i - 1, j, k), Y_(u2, i - 1, j + 1, k), Y_(u2, i, j + 1, k), Y_(u2, i, j, k),
-             LU2(i - 1, j, k), LU2(i - 1, j + 1, k), LU2(i, j + 1, k), LU2(i, j, k),
-             i - 1, j + 1, k, i, j + 1, k, i - 1, j - 1, k, i, j - 1, k)
-        if (zterms)
-            CORR(c1, Z_(u3, i - 1, j, k), Z_(u3, i - 1, j, k + 1), Z_(u3, i, j, k + 1), Z_(u3, i, j, k),
-                 LU3(i - 1, j, k), LU3(i - 1, j, k + 1), LU3(i, j, k + 1), LU3(i, j, k),
-                 i - 1, j, k + 1, i, j, k + 1, i - 1, j, k - 1, i, j, k - 1)
-        adj_face_pseudo(L, X_(a.v1, i, j, k), X_(a.f1, i, j, k), X_(v1, i, j, k), AXV(i - 1, j, k),
-                        AXV(i, j, k), X_(u1, i, j, k), hm, LPS(i - 1, j, k), LPS(i, j, k),
-                        LU1(i, j, k), c0, c1, zterms, a.uf);
-    }
-    if (valid && j > 1) {
-        const R hm = R(1) / (R(0.5) * (S_(h, i, j - 1, k) + S_(h, i, j, k)));
-        Corr c0, c1;
-        CORR(c0, X_(u1, i, j - 1, k), X_(u1, i, j, k), X_(u1, i + 1, j, k), X_(u1, i + 1, j - 1, k),
-             LU1(i, j - 1, k), LU1(i, j, k), LU1(i + 1, j, k), LU1(i + 1, j - 1, k),
-             i + 1, j - 1, k, i + 1, j, k, i - 1, j - 1, k, i - 1, j, k)
-        if (zterms)
-            CORR(c1, Z_(u3, i, j - 1, k), Z_(u3, i, j, k), Z_(u3, i, j, k + 1), Z_(u3, i, j - 1, k + 1),
-                 LU3(i, j - 1, k), LU3(i, j, k), LU3(i, j, k + 1), LU3(i, j - 1, k + 1),
-                 i, j - 1, k + 1, i, j, k + 1, i, j - 1, k - 1, i, j, k - 1)
-        adj_face_pseudo(L, Y_(a.v2, i, j, k), Y_(a.f2, i, j, k), Y_(v2, i, j, k), AXV(i, j - 1, k),
-                        AXV(i, j, k), Y_(u2, i, j, k), hm, LPS(i, j - 1, k), LPS(i, j, k),
-                        LU2(i, j, k), c0, c1, zterms, a.uf);
-    }
-    if (valid && !gnd) {
-        const R hm = R(1) / (R(0.5) * (S_(h, i, j, k - 1) + S_(h, i, j, k)));
-        Corr c0, c1;
-        CORR(c0, X_(u1, i, j, k - 1), X_(u1, i, j, k), X_(u1, i + 1, j, k), X_(u1, i + 1, j, k - 1),
-             LU1(i, j, k - 1), LU1(i, j, k), LU1(i + 1, j, k), LU1(i + 1, j, k - 1),
-             i + 1, j, k - 1, i + 1, j, k, i - 1, j, k - 1, i - 1, j, k)
-        CORR(c1, Y_(u2, i, j, k - 1), Y_(u2, i, j + 1, k - 1), Y_(u2, i, j + 1, k), Y_(u2, i, j, k),
-             LU2(i, j, k - 1), LU2(i, j + 1, k - 1), LU2(i, j + 1, k), LU2(i, j, k),
-             i, j + 1, k - 1, i, j + 1, k, i, j - 1, k - 1, i, j - 1, k)
-        adj_face_pseudo(L, Z_(a.v3, i, j, k), Z_(a.f3, i, j, k), Z_(v3, i, j, k), AXV(i, j, k - 1),
-                        AXV(i, j, k), Z_(u3, i, j, k), hm, LPS(i, j, k - 1), LPS(i, j, k),
-                        LU3(i, j, k), c0, c1, true, a.uf);
-    }
+    const int ih = g.ih;
+    constexpr int PS = (PBY + 2) * (PBX + 2), US = (PBY + 1) * (PBX + 1);  // one plane
+    // adds one plane of a window to the global array and clears it
+    auto flush_psi = [&](int kk) {
+        R* const w = L + (kk % 3) * PS;
+        for (int q = tid; q < PS; q += PBX * PBY) {
+            const R v = w[q];
+            if (v == R(0)) continue;
+            w[q] = R(0);
+            const int ii = i0 - 1 + q % (PBX + 2), jj = j0 - 1 + q / (PBX + 2);
+            if (ii <= g.np + ih && jj <= g.mp + ih) acc(SP(a.psi, ii, jj, kk), v);
+        }
+    };
+    auto flush_u12 = [&](int kk) {
+        R* const w1 = L + L_U1 + (kk % 2) * US;
+        R* const w2 = L + L_U2 + (kk % 2) * US;
+        for (int q = tid; q < US; q += PBX * PBY) {
+            const int a0 = q % (PBX + 1), b0 = q / (PBX + 1);
+            R v = w1[q];
+            if (v != R(0)) {
+                w1[q] = R(0);
+                const int ii = i0 + a0, jj = j0 - 1 + b0;
+                if (ii <= g.np + 1 + ih && jj <= g.mp + ih) acc(XP(ub1, ii, jj, kk), v);
+            }
+            v = w2[q];
+            if (v != R(0)) {
+                w2[q] = R(0);
+                const int ii = i0 - 1 + a0, jj = j0 + b0;
+                if (ii <= g.np + ih && jj <= g.mp + 1 + ih) acc(YP(ub2, ii, jj, kk), v);
+            }
+        }
+    };
+    auto flush_u3 = [&](int kk) {
+        R* const w3 = L + L_U3 + (kk % 2) * US;
+        for (int q = tid; q < US; q += PBX * PBY) {
+            const R v = w3[q];
+            if (v == R(0)) continue;
+            w3[q] = R(0);
+            const int ii = i0 - 1 + q % (PBX + 1), jj = j0 - 1 + q / (PBX + 1);
+            if (ii <= g.np + ih && jj <= g.mp + ih) acc(ZP(ub3, ii, jj, kk), v);
+        }
+    };
+    for (int k = kb0; k <= kb1; k++) {
+        const bool gnd = k == 1, sky = k == g.lp;
+        const bool zterms = !(gnd || sky);
+#define LPS(ii, jj, kk) (((kk) % 3) * PS + ((jj) - j0 + 1) * (PBX + 2) + ((ii) - i0 + 1))
+#define LU1(ii, jj, kk) (L_U1 + ((kk) % 2) * US + ((jj) - j0 + 1) * (PBX + 1) + ((ii) - i0))
+#define LU2(ii, jj, kk) (L_U2 + ((kk) % 2) * US + ((jj) - j0) * (PBX + 1) + ((ii) - i0 + 1))
+#define LU3(ii, jj, kk) (L_U3 + ((kk) % 2) * US + ((jj) - j0 + 1) * (PBX + 1) + ((ii) - i0 + 1))
+    #define AXV(ii, jj, kk) S_(psi, ii, jj, kk)
+    #define CORR(c, F1, F2, F3, F4, BF1, BF2, BF3, BF4, I0, J0, K0, I1, J1, K1, I2, J2, K2, I3, J3, K3) \
+        {                                                                                          \
+            c.bf[0] = BF1; c.bf[1] = BF2; c.bf[2] = BF3; c.bf[3] = BF4;                            \
+            c.b = F1 + F2 + F3 + F4;                                                               \
+            const R v0_ = AXV(I0, J0, K0), v1_ = AXV(I1, J1, K1);                                  \
+            const R v2_ = AXV(I2, J2, K2), v3_ = AXV(I3, J3, K3);                                  \
+            c.sg[0] = sgn(v0_); c.sg[1] = sgn(v1_); c.sg[2] = sgn(v2_); c.sg[3] = sgn(v3_);        \
+            c.cb[0] = LPS(I0, J0, K0); c.cb[1] = LPS(I1, J1, K1);                                  \
+            c.cb[2] = LPS(I2, J2, K2); c.cb[3] = LPS(I3, J3, K3);                                  \
+            c.y1 = (fabs(v0_) + fabs(v1_)) - (fabs(v2_) + fabs(v3_));                              \
+            c.y2 = fabs(v0_) + fabs(v1_) + fabs(v2_) + fabs(v3_) + ep;                             \
+            c.iy2 = R(1) / c.y2;                                                                   \
+        }
+        if (valid && i > 1) {
+            const R hm = R(1) / (R(0.5) * (S_(h, i - 1, j, k) + S_(h, i, j, k)));
+            Corr c0, c1;
+            CORR(c0, Y_(u2, i - 1, j, k), Y_(u2, i - 1, j + 1, k), Y_(u2, i, j + 1, k), Y_(u2, i, j, k),
+                 LU2(i - 1, j, k), LU2(i - 1, j + 1, k), LU2(i, j + 1, k), LU2(i, j, k),
+                 i - 1, j + 1, k, i, j + 1, k, i - 1, j - 1, k, i, j - 1, k)
+            if (zterms)
+                CORR(c1, Z_(u3, i - 1, j, k), Z_(u3, i - 1, j, k + 1), Z_(u3, i, j, k + 1), Z_(u3, i, j, k),
+                     LU3(i - 1, j, k), LU3(i - 1, j, k + 1), LU3(i, j, k + 1), LU3(i, j, k),
+                     i - 1, j, k + 1, i, j, k + 1, i - 1, j, k - 1, i, j, k - 1)
+            adj_face_pseudo(L, X_(a.v1, i, j, k), X_(a.f1, i, j, k), X_(v1, i, j, k), AXV(i - 1, j, k),
+                            AXV(i, j, k), X_(u1, i, j, k), hm, LPS(i - 1, j, k), LPS(i, j, k),
+                            LU1(i, j, k), c0, c1, zterms, a.uf);
+        }
+        if (valid && j > 1) {
+            const R hm = R(1) / (R(0.5) * (S_(h, i, j - 1, k) + S_(h, i, j, k)));
+            Corr c0, c1;
+            CORR(c0, X_(u1, i, j - 1, k), X_(u1, i, j, k), X_(u1, i + 1, j, k), X_(u1, i + 1, j - 1, k),
+                 LU1(i, j - 1, k), LU1(i, j, k), LU1(i + 1, j, k), LU1(i + 1, j - 1, k),
+                 i + 1, j - 1, k, i + 1, j, k, i - 1, j - 1, k, i - 1, j, k)
+            if (zterms)
+                CORR(c1, Z_(u3, i, j - 1, k), Z_(u3, i, j, k), Z_(u3, i, j, k + 1), Z_(u3, i, j - 1, k + 1),
+                     LU3(i, j - 1, k), LU3(i, j, k), LU3(i, j, k + 1), LU3(i, j - 1, k + 1),
+                     i, j - 1, k + 1, i, j, k + 1, i, j - 1, k - 1, i, j, k - 1)
+            adj_face_pseudo(L, Y_(a.v2, i, j, k), Y_(a.f2, i, j, k), Y_(v2, i, j, k), AXV(i, j - 1, k),
+                            AXV(i, j, k), Y_(u2, i, j, k), hm, LPS(i, j - 1, k), LPS(i, j, k),
+                            LU2(i, j, k), c0, c1, zterms, a.uf);
+        }
+        if (valid && !gnd) {
+            const R hm = R(1) / (R(0.5) * (S_(h, i, j, k - 1) + S_(h, i, j, k)));
+            Corr c0, c1;
+            CORR(c0, X_(u1, i, j, k - 1), X_(u1, i, j, k), X_(u1, i + 1, j, k), X_(u1, i + 1, j, k - 1),
+                 LU1(i, j, k - 1), LU1(i, j, k), LU1(i + 1, j, k), LU1(i + 1, j, k - 1),
+                 i + 1, j, k - 1, i + 1, j, k, i - 1, j, k - 1, i - 1, j, k)
+            CORR(c1, Y_(u2, i, j, k - 1), Y_(u2, i, j + 1, k - 1), Y_(u2, i, j + 1, k), Y_(u2, i, j, k),
+                 LU2(i, j, k - 1), LU2(i, j + 1, k - 1), LU2(i, j + 1, k), LU2(i, j, k),
+                 i, j + 1, k - 1, i, j + 1, k, i, j - 1, k - 1, i, j - 1, k)
+            adj_face_pseudo(L, Z_(a.v3, i, j, k), Z_(a.f3, i, j, k), Z_(v3, i, j, k), AXV(i, j, k - 1),
+                            AXV(i, j, k), Z_(u3, i, j, k), hm, LPS(i, j, k - 1), LPS(i, j, k),
+                            LU3(i, j, k), c0, c1, true, a.uf);
+        }
 #undef CORR
 #undef AXV
 #undef LPS
 #undef LU1
 #undef LU2
 #undef LU3
-    __syncthreads();
-    // flush the regions; entries outside the arrays stay zero and are skipped
-    const int ih = g.ih;
-    for (int q = tid; q < LP_N; q += PBX * PBY) {
-        const R v = L[q];
-        if (v == R(0)) continue;
-        const int ii = i0 - 1 + q % (PBX + 2), jj = j0 - 1 + (q / (PBX + 2)) % (PBY + 2),
-                  kk = k - 1 + q / ((PBX + 2) * (PBY + 2));
-        if (ii <= g.np + ih && jj <= g.mp + ih) acc(SP(a.psi, ii, jj, kk), v);
+        __syncthreads();
+        flush_psi(k - 1);
+        flush_u12(k - 1);
+        flush_u3(k);
+        __syncthreads();
     }
-    for (int q = tid; q < LU_N; q += PBX * PBY) {
-        const int a0 = q % (PBX + 1), b0 = (q / (PBX + 1)) % (PBY + 1), c0 = q / ((PBX + 1) * (PBY + 1));
-        R v = L[L_U1 + q];
-        if (v != R(0)) {
-            const int ii = i0 + a0, jj = j0 - 1 + b0, kk = k - 1 + c0;
-            if (ii <= g.np + 1 + ih && jj <= g.mp + ih) acc(XP(ub1, ii, jj, kk), v);
-        }
-        v = L[L_U2 + q];
-        if (v != R(0)) {
-            const int ii = i0 - 1 + a0, jj = j0 + b0, kk = k - 1 + c0;
-            if (ii <= g.np + ih && jj <= g.mp + 1 + ih) acc(YP(ub2, ii, jj, kk), v);
-        }
-        v = L[L_U3 + q];
-        if (v != R(0)) {
-            const int ii = i0 - 1 + a0, jj = j0 - 1 + b0, kk = k + c0;
-            if (ii <= g.np + ih && jj <= g.mp + ih) acc(ZP(ub3, ii, jj, kk), v);
-        }
-    }
+    // the planes still open after the last one
+    flush_psi(kb1);
+    flush_psi(kb1 + 1);
+    flush_u12(kb1);
+    flush_u3(kb1 + 1);
 }
 
 // ---- upwind (upwind3d-gpubc.f90:101-198) and the sum of both uses of x -----
@@ -783,7 +814,8 @@ int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double*
                                                 (const R*)t.fns[0], (const R*)t.fns[1],
                                                 (const R*)t.fns[2], (const R*)t.cp,
                                                 (const R*)t.cn, a);
-    k_adj_pseudo<<<dim3((g.np + PBX - 1) / PBX, (g.mp + PBY - 1) / PBY, g.lp), dim3(PBX, PBY, 1), 0,
+    k_adj_pseudo<<<dim3((g.np + PBX - 1) / PBX, (g.mp + PBY - 1) / PBY, (g.lp + PKB - 1) / PKB),
+                   dim3(PBX, PBY, 1), 0,
                    s>>>(g, u1, u2, u3, (const R*)t.psi, h,
                                                (const R*)t.v[0], (const R*)t.v[1],
                                                (const R*)t.v[2], ubar1, ubar2, ubar3, a);

@@ -65,7 +65,8 @@ struct FusedStd {
                            const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
                            const T* bcy, hipStream_t s);
     // The limited step with open walls and ih = 1 that also records the
-    // adjoint's tape (StdTape); xout as step.
+    // adjoint's tape (StdTape); xout receives the interior and the edge
+    // copies, its other halo cells are left as they were.
     static hipError_t record(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
                              const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
                              const StdTape<T>& tp, hipStream_t s);

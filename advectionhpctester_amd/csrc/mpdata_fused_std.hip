@@ -706,11 +706,10 @@ hipError_t FusedStd<T>::record(const Geo& g, const T* x, T* xout, const T* u1, c
     const Off o = make_off(g);
     int cfg = 0, cl = 64;
     choose_tiling<T>(g, cfg, cl);
-    hipError_t e = launch_std<T, 32, 16, 2, true, false, true>(g, o, cl, x, xout, u1, u2, u3, rhr,
-                                                               h, bcx, bcy, s, tp);
-    if (e != hipSuccess) return e;
-    k_copy_untouched_halo<T><<<256, 256, 0, s>>>(g, x, xout);
-    return hipGetLastError();
+    // xout: the interior and the edge copies; the other halo cells are left
+    // alone (the recorded step defines only the interior)
+    return launch_std<T, 32, 16, 2, true, false, true>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx,
+                                                       bcy, s, tp);
 }
 
 template struct FusedStd<double>;

@@ -361,9 +361,15 @@ def test_tape_replaced_between_forward_and_backward(mpd):
     obs = to_dev(np.random.default_rng(41).uniform(-1, 2, c["x"].shape))
     grads = []
     with mpd.Domain(*dims) as dom:
+        xp = d["x"].clone()
+        for _ in range(7):
+            dom.step(xp, d["u1"], d["u2"], d["u3"], *rest, variant=mpd.STANDARD)
         for replace in (False, True):
             leaves = [d[k].clone().requires_grad_(True) for k in ("x", "u1", "u2", "u3")]
             xT = adjoint.advect(*leaves, dom, *rest, 7, ckpt=3)
+            # the taped forward (its last steps recorded) gives the plain
+            # steps' whole array, halo included
+            assert bits_equal(to_host(xT.detach()), to_host(xp))
             if replace:
                 print("tape replaced before the backward", flush=True)
                 dom.adj_record(0, d["x"], torch.empty_like(d["x"]), d["u1"], d["u2"], d["u3"],
