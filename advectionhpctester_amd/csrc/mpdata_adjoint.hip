@@ -126,39 +126,40 @@ __global__ void k_prep_x(Geo g, const R* __restrict__ x, const R* __restrict__ b
 // the block.
 constexpr int BX = 64, BY = 4;
 
-struct Region {   // LDS slot of (x, y, z) = off + ((z - z0) * ny + (y - y0)) * nx + (x - x0)
-    int x0, y0, z0, nx, ny, nz, off;
-    __device__ int at(int x, int y, int z) const { return off + ((z - z0) * ny + (y - y0)) * nx + (x - x0); }
-    __device__ int size() const { return nx * ny * nz; }
+// A rolling LDS window of a block over (x, y) planes: plane z sits in slot
+// z mod nz, nx x ny entries from (x0, y0).  Blocks march FKB planes k; a
+// plane that no later k of the block reaches is added to the global array
+// with one atomic per entry and cleared for reuse (flush_plane).
+constexpr int FKB = 4;
+struct Region {
+    int x0, y0, nx, ny, nz, off;
+    __device__ int at(int x, int y, int z) const { return off + ((z % nz) * ny + (y - y0)) * nx + (x - x0); }
+    __device__ int plane() const { return nx * ny; }
 };
 
 template <int KIND>
-__device__ void flush_region(const Geo& g, const R* L, const Region& r, R* a, int tid) {
+__device__ void flush_plane(const Geo& g, R* L, const Region& r, int z, R* a, int tid) {
     const int ih = g.ih;
     const int xmax = g.np + ih + (KIND == FACEX), ymax = g.mp + ih + (KIND == FACEY);
-    const int zmax = g.lp + ih + (KIND == FACEZ);
-    for (int q = tid; q < r.size(); q += BX * BY) {
-        const R v = L[r.off + q];
+    R* const w = L + r.at(r.x0, r.y0, z);
+    for (int q = tid; q < r.plane(); q += BX * BY) {
+        const R v = w[q];
         if (v == R(0)) continue;
-        const int x = r.x0 + q % r.nx, y = r.y0 + (q / r.nx) % r.ny, z = r.z0 + q / (r.nx * r.ny);
-        if (x <= xmax && y <= ymax && z <= zmax) atomicAdd(&a[off_raw<KIND>(g, x, y, z)], v);
+        w[q] = R(0);
+        const int x = r.x0 + q % r.nx, y = r.y0 + q / r.nx;
+        if (x <= xmax && y <= ymax) atomicAdd(&a[off_raw<KIND>(g, x, y, z)], v);
     }
 }
 
 // ---- limited donor fluxes (:372-407), one face ----------------------------
-// fb = adjoint of the flux; m/c = upstream/downstream cells
-__device__ __forceinline__ void adj_face_flux(R* L, R fb, R v, R vl, R pm, R pc, R rm, R rc,
+// fb = adjoint of the flux; m/c = upstream/downstream cells.  The limited
+// velocity vl = slimv(v, a1, a2, pm, pc) is formed again from the tape's V,
+// cp, cn and xant, with the forward's operations (its sign of a zero result
+// may differ, which no branch below sees).
+__device__ __forceinline__ void adj_face_flux(R* L, R fb, R v, R pm, R pc, R rm, R rc,
                                               R cp_c, R cn_m, R cp_m, R cn_c, int psib_m,
                                               int psib_c, int cpb_c, int cnb_m, int cpb_m,
                                               int cnb_c, R* vbar) {
-    const R y1 = pm / rm, y2 = pc / rc;
-    // f = donor(y1, y2, vl)
-    const R y1b = fb * (vl > R(0) ? vl : R(0));
-    const R y2b = fb * (vl < R(0) ? vl : R(0));
-    const R vlb = fb * (ind_pos(vl) * y1 + ind_neg(vl) * y2);
-    atomicAdd(&L[psib_m], y1b / rm);
-    atomicAdd(&L[psib_c], y2b / rc);
-    // vl = slimv(v, a1, a2, pm, pc)
     const R Pm = copysign(R(1), pm) > R(0) ? R(1) : R(0);
     const R Nm = copysign(R(1), -pm) > R(0) ? R(1) : R(0);
     const R Pc = copysign(R(1), pc) > R(0) ? R(1) : R(0);
@@ -169,6 +170,15 @@ __device__ __forceinline__ void adj_face_flux(R* L, R fb, R v, R vl, R pm, R pc,
     const R a2 = m2 < cn_c ? m2 : cn_c;
     const R ppv = v > R(0) ? v : R(0), pnv = v < R(0) ? -v : R(0);
     const R K1 = a1 * Pm + a2 * Nm, K2 = a2 * Pc + a1 * Nc;
+    const R vl = ppv * K1 - pnv * K2;
+    const R y1 = pm / rm, y2 = pc / rc;
+    // f = donor(y1, y2, vl)
+    const R y1b = fb * (vl > R(0) ? vl : R(0));
+    const R y2b = fb * (vl < R(0) ? vl : R(0));
+    const R vlb = fb * (ind_pos(vl) * y1 + ind_neg(vl) * y2);
+    atomicAdd(&L[psib_m], y1b / rm);
+    atomicAdd(&L[psib_c], y2b / rc);
+    // vl = slimv(v, a1, a2, pm, pc)
     *vbar = vlb * (ind_pos(v) * K1 + ind_neg(v) * K2);
     const R a1b = vlb * (ppv * Pm - pnv * Nc);
     const R a2b = vlb * (ppv * Nm - pnv * Pc);
@@ -188,31 +198,30 @@ __device__ __forceinline__ void adj_face_flux(R* L, R fb, R v, R vl, R pm, R pc,
 __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict__ v1,
                                                       const R* __restrict__ v2,
                                                       const R* __restrict__ v3,
-                                                      const R* __restrict__ vl1,
-                                                      const R* __restrict__ vl2,
-                                                      const R* __restrict__ vl3,
                                                       const R* __restrict__ cp,
                                                       const R* __restrict__ cn,
                                                       const R* __restrict__ psi,
                                                       const R* __restrict__ rhr, Adj a) {
-    // targets: the cell and its three low neighbours
+    // targets: the cell and its three low neighbours (planes k-1, k)
     constexpr int N = (BX + 1) * (BY + 1) * 2;
     __shared__ R L[3 * N];
     const int tid = threadIdx.y * BX + threadIdx.x;
-    const int i0 = 1 + blockIdx.x * BX, j0 = 1 + blockIdx.y * BY, k = 1 + blockIdx.z;
-    const Region rp{i0 - 1, j0 - 1, k - 1, BX + 1, BY + 1, 2, 0};
-    const Region rcp{i0 - 1, j0 - 1, k - 1, BX + 1, BY + 1, 2, N};
-    const Region rcn{i0 - 1, j0 - 1, k - 1, BX + 1, BY + 1, 2, 2 * N};
+    const int i0 = 1 + blockIdx.x * BX, j0 = 1 + blockIdx.y * BY;
+    const int kb0 = 1 + blockIdx.z * FKB, kb1 = min(g.lp, kb0 + FKB - 1);
+    const Region rp{i0 - 1, j0 - 1, BX + 1, BY + 1, 2, 0};
+    const Region rcp{i0 - 1, j0 - 1, BX + 1, BY + 1, 2, N};
+    const Region rcn{i0 - 1, j0 - 1, BX + 1, BY + 1, 2, 2 * N};
     for (int q = tid; q < 3 * N; q += BX * BY) L[q] = R(0);
     __syncthreads();
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    for (int k = kb0; k <= kb1; k++) {
     if (i <= g.np && j <= g.mp) {
         const bool sky = k == g.lp;
         const R pc = S_(psi, i, j, k), rc = S_(rhr, i, j, k);
         const R sc = S_(a.sum, i, j, k);
         if (i > 1) {
             const R fb = S_(a.sum, i - 1, j, k) - sc;
-            adj_face_flux(L, fb, X_(v1, i, j, k), X_(vl1, i, j, k), S_(psi, i - 1, j, k), pc,
+            adj_face_flux(L, fb, X_(v1, i, j, k), S_(psi, i - 1, j, k), pc,
                           S_(rhr, i - 1, j, k), rc, S_(cp, i, j, k), S_(cn, i - 1, j, k),
                           S_(cp, i - 1, j, k), S_(cn, i, j, k), rp.at(i - 1, j, k), rp.at(i, j, k),
                           rcp.at(i, j, k), rcn.at(i - 1, j, k), rcp.at(i - 1, j, k),
@@ -220,7 +229,7 @@ __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict
         }
         if (j > 1) {
             const R fb = S_(a.sum, i, j - 1, k) - sc;
-            adj_face_flux(L, fb, Y_(v2, i, j, k), Y_(vl2, i, j, k), S_(psi, i, j - 1, k), pc,
+            adj_face_flux(L, fb, Y_(v2, i, j, k), S_(psi, i, j - 1, k), pc,
                           S_(rhr, i, j - 1, k), rc, S_(cp, i, j, k), S_(cn, i, j - 1, k),
                           S_(cp, i, j - 1, k), S_(cn, i, j, k), rp.at(i, j - 1, k), rp.at(i, j, k),
                           rcp.at(i, j, k), rcn.at(i, j - 1, k), rcp.at(i, j - 1, k),
@@ -230,7 +239,7 @@ __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict
             // the ground cell counts its upper face twice, the sky cell its lower
             const R c1 = k - 1 == 1 ? R(2) : R(1), c2 = sky ? R(2) : R(1);
             const R fb = c1 * S_(a.sum, i, j, k - 1) - c2 * sc;
-            adj_face_flux(L, fb, Z_(v3, i, j, k), Z_(vl3, i, j, k), S_(psi, i, j, k - 1), pc,
+            adj_face_flux(L, fb, Z_(v3, i, j, k), S_(psi, i, j, k - 1), pc,
                           S_(rhr, i, j, k - 1), rc, S_(cp, i, j, k), S_(cn, i, j, k - 1),
                           S_(cp, i, j, k - 1), S_(cn, i, j, k), rp.at(i, j, k - 1), rp.at(i, j, k),
                           rcp.at(i, j, k), rcn.at(i, j, k - 1), rcp.at(i, j, k - 1),
@@ -238,9 +247,14 @@ __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict
         }
     }
     __syncthreads();
-    flush_region<SCAL>(g, L, rp, a.psi, tid);
-    flush_region<SCAL>(g, L, rcp, a.cp, tid);
-    flush_region<SCAL>(g, L, rcn, a.cn, tid);
+    flush_plane<SCAL>(g, L, rp, k - 1, a.psi, tid);
+    flush_plane<SCAL>(g, L, rcp, k - 1, a.cp, tid);
+    flush_plane<SCAL>(g, L, rcn, k - 1, a.cn, tid);
+    __syncthreads();
+    }
+    flush_plane<SCAL>(g, L, rp, kb1, a.psi, tid);
+    flush_plane<SCAL>(g, L, rcp, kb1, a.cp, tid);
+    flush_plane<SCAL>(g, L, rcn, kb1, a.cn, tid);
 }
 
 // ---- limiter (:260-347) ----------------------------------------------------
@@ -260,25 +274,28 @@ __device__ __forceinline__ void pick_min(Pick& p, R v, int src, int di, int dj, 
 __global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restrict__ x0,
                                                          const R* __restrict__ psi,
                                                          const R* __restrict__ h,
-                                                         const R* __restrict__ w1,
-                                                         const R* __restrict__ w2,
-                                                         const R* __restrict__ w3,
+                                                         const R* __restrict__ v1,
+                                                         const R* __restrict__ v2,
+                                                         const R* __restrict__ v3,
                                                          const R* __restrict__ cp,
                                                          const R* __restrict__ cn, Adj a) {
-    // targets: psi and x0 within one cell in every direction, the six faces
+    // targets: psi and x0 within one cell in every direction (planes
+    // k-1..k+1), the six faces (x/y faces of plane k, z-faces k and k+1)
     constexpr int NC = (BX + 2) * (BY + 2) * 3;
     constexpr int N1 = (BX + 1) * BY, N2 = BX * (BY + 1), N3 = BX * BY * 2;
     __shared__ R L[2 * NC + N1 + N2 + N3];
     const int tid = threadIdx.y * BX + threadIdx.x;
-    const int i0 = 1 + blockIdx.x * BX, j0 = 1 + blockIdx.y * BY, k = 1 + blockIdx.z;
-    const Region rp{i0 - 1, j0 - 1, k - 1, BX + 2, BY + 2, 3, 0};
-    const Region rx{i0 - 1, j0 - 1, k - 1, BX + 2, BY + 2, 3, NC};
-    const Region r1{i0, j0, k, BX + 1, BY, 1, 2 * NC};
-    const Region r2{i0, j0, k, BX, BY + 1, 1, 2 * NC + N1};
-    const Region r3{i0, j0, k, BX, BY, 2, 2 * NC + N1 + N2};
+    const int i0 = 1 + blockIdx.x * BX, j0 = 1 + blockIdx.y * BY;
+    const int kb0 = 1 + blockIdx.z * FKB, kb1 = min(g.lp, kb0 + FKB - 1);
+    const Region rp{i0 - 1, j0 - 1, BX + 2, BY + 2, 3, 0};
+    const Region rx{i0 - 1, j0 - 1, BX + 2, BY + 2, 3, NC};
+    const Region r1{i0, j0, BX + 1, BY, 1, 2 * NC};
+    const Region r2{i0, j0, BX, BY + 1, 1, 2 * NC + N1};
+    const Region r3{i0, j0, BX, BY, 2, 2 * NC + N1 + N2};
     for (int q = tid; q < 2 * NC + N1 + N2 + N3; q += BX * BY) L[q] = R(0);
     __syncthreads();
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+    for (int k = kb0; k <= kb1; k++) {
     if (i <= g.np && j <= g.mp) {
         const bool gnd = k == 1, sky = k == g.lp;
         const R ep = Eps<R>::v;
@@ -299,12 +316,21 @@ __global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restr
 #undef ACC
         if (!(mx.val > mxo.val)) mx = mxo;   // mx = max2(mx, mxo)
         if (!(mn.val < mno.val)) mn = mno;   // mn = min2(mn, mno)
-        const R a1p = X_(w1, i + 1, j, k), a1 = X_(w1, i, j, k);
-        const R a2p = Y_(w2, i, j + 1, k), a2 = Y_(w2, i, j, k);
+        // the limiter's inputs, the donor fluxes of the pseudo-velocities
+        // (:182-202), formed again from the tape's V and xant (zero V on the
+        // x/y walls gives zero fluxes there)
+        const R pc_ = S_(psi, i, j, k);
+        const R a1p = donor(pc_, S_(psi, i + 1, j, k), X_(v1, i + 1, j, k));
+        const R a1 = donor(S_(psi, i - 1, j, k), pc_, X_(v1, i, j, k));
+        const R a2p = donor(pc_, S_(psi, i, j + 1, k), Y_(v2, i, j + 1, k));
+        const R a2 = donor(S_(psi, i, j - 1, k), pc_, Y_(v2, i, j, k));
         R a3p, a3;
-        if (gnd) { a3p = Z_(w3, i, j, k + 1); a3 = -a3p; }
-        else if (sky) { a3 = Z_(w3, i, j, k); a3p = -a3; }
-        else { a3p = Z_(w3, i, j, k + 1); a3 = Z_(w3, i, j, k); }
+        if (gnd) { a3p = donor(pc_, S_(psi, i, j, k + 1), Z_(v3, i, j, k + 1)); a3 = -a3p; }
+        else if (sky) { a3 = donor(S_(psi, i, j, k - 1), pc_, Z_(v3, i, j, k)); a3p = -a3; }
+        else {
+            a3p = donor(pc_, S_(psi, i, j, k + 1), Z_(v3, i, j, k + 1));
+            a3 = donor(S_(psi, i, j, k - 1), pc_, Z_(v3, i, j, k));
+        }
         const R hc = S_(h, i, j, k);
         const R qp = (a1p < R(0) ? -a1p : R(0)) + (a1 > R(0) ? a1 : R(0)) + (a2p < R(0) ? -a2p : R(0)) +
                      (a2 > R(0) ? a2 : R(0)) + (a3p < R(0) ? -a3p : R(0)) + (a3 > R(0) ? a3 : R(0)) + ep;
@@ -340,11 +366,18 @@ __global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restr
         }
     }
     __syncthreads();
-    flush_region<SCAL>(g, L, rp, a.psi, tid);
-    flush_region<SCAL>(g, L, rx, a.x0, tid);
-    flush_region<FACEX>(g, L, r1, a.f1, tid);
-    flush_region<FACEY>(g, L, r2, a.f2, tid);
-    flush_region<FACEZ>(g, L, r3, a.f3, tid);
+    flush_plane<SCAL>(g, L, rp, k - 1, a.psi, tid);
+    flush_plane<SCAL>(g, L, rx, k - 1, a.x0, tid);
+    flush_plane<FACEX>(g, L, r1, k, a.f1, tid);
+    flush_plane<FACEY>(g, L, r2, k, a.f2, tid);
+    flush_plane<FACEZ>(g, L, r3, k, a.f3, tid);
+    __syncthreads();
+    }
+    flush_plane<SCAL>(g, L, rp, kb1, a.psi, tid);
+    flush_plane<SCAL>(g, L, rp, kb1 + 1, a.psi, tid);
+    flush_plane<SCAL>(g, L, rx, kb1, a.x0, tid);
+    flush_plane<SCAL>(g, L, rx, kb1 + 1, a.x0, tid);
+    flush_plane<FACEZ>(g, L, r3, kb1 + 1, a.f3, tid);
 }
 
 // ---- pseudo-velocities and first-order fluxes (:72-202) --------------------
@@ -686,9 +719,7 @@ int ensure_slot(mpdata_ctx* c, TapeSlot& t) {
         (rc = ensure(c, &t.cn, SCAL, 8)))
         return rc;
     for (int d = 0; d < 3; d++)
-        if ((rc = ensure(c, &t.v[d], FACEX + d, 8)) || (rc = ensure(c, &t.fns[d], FACEX + d, 8)) ||
-            (rc = ensure(c, &t.vl[d], FACEX + d, 8)))
-            return rc;
+        if ((rc = ensure(c, &t.v[d], FACEX + d, 8))) return rc;
     return MPDATA_OK;
 }
 
@@ -722,19 +753,12 @@ int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x
     if ((const char*)x_in < (const char*)x_out + sb && (const char*)x_out < (const char*)x_in + sb)
         return fail(MPDATA_EINVAL, "x_in and x_out overlap");
     Scratch& S = c->prim;
-    for (int d = 0; d < 3; d++)
-        if ((rc = ensure(c, &S.f[d], FACEX + d, 8))) return rc;
     TapeSlot& t = c->tape[slot];
     hipStream_t s = (hipStream_t)stream;
     using St = Stages<R>;
     R *xb = (R*)t.xb, *x0 = (R*)t.x0, *psi = (R*)t.psi, *cp = (R*)t.cp, *cn = (R*)t.cn;
-    R *v[3], *fns[3], *vl[3], *f[3];
-    for (int d = 0; d < 3; d++) {
-        v[d] = (R*)t.v[d];
-        fns[d] = (R*)t.fns[d];
-        vl[d] = (R*)t.vl[d];
-        f[d] = (R*)S.f[d];
-    }
+    R* v[3];
+    for (int d = 0; d < 3; d++) v[d] = (R*)t.v[d];
     if (c->path != MPDATA_PATH_STAGES) {
         // the fused standard step, writing the tape as it goes
         StdTape<R> tp;
@@ -743,15 +767,22 @@ int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x
         tp.psi = psi;
         tp.cp = cp;
         tp.cn = cn;
-        for (int d = 0; d < 3; d++) {
-            tp.v[d] = v[d];
-            tp.w[d] = fns[d];
-            tp.vl[d] = vl[d];
-        }
+        for (int d = 0; d < 3; d++) tp.v[d] = v[d];
         HIPCHK(FusedStd<R>::record(g, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, tp, s));
         return MPDATA_OK;
     }
-    // the stage sequence of the standard non-oscillatory step
+    // the stage sequence of the standard non-oscillatory step; the donor
+    // fluxes of V, the limited velocities and the fluxes go to the context's
+    // scratch (the adjoint forms the first two again from the tape)
+    R *fns[3], *vl[3], *f[3];
+    for (int d = 0; d < 3; d++) {
+        if ((rc = ensure(c, &S.fns[d], FACEX + d, 8)) || (rc = ensure(c, &S.ul[d], FACEX + d, 8)) ||
+            (rc = ensure(c, &S.f[d], FACEX + d, 8)))
+            return rc;
+        fns[d] = (R*)S.fns[d];
+        vl[d] = (R*)S.ul[d];
+        f[d] = (R*)S.f[d];
+    }
     k_prep_x<<<dim3((g.np + 2 * g.ih + 63) / 64, (g.mp + 2 * g.ih + 3) / 4, g.lp + 2 * g.ih),
                dim3(64, 4, 1), 0, s>>>(g, x_in, bcx, bcy, xb, x0);
     HIPCHK(hipGetLastError());
@@ -806,13 +837,12 @@ int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double*
             g, z, xbar, rhr, h, a);
     }
     HIPCHK(hipGetLastError());
-    k_adj_flux<<<dim3((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, g.lp), dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.v[0], (const R*)t.v[1],
-                                             (const R*)t.v[2], (const R*)t.vl[0],
-                                             (const R*)t.vl[1], (const R*)t.vl[2],
+    k_adj_flux<<<dim3((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, (g.lp + FKB - 1) / FKB), dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.v[0], (const R*)t.v[1],
+                                             (const R*)t.v[2],
                                              (const R*)t.cp, (const R*)t.cn, (const R*)t.psi, rhr, a);
-    k_adj_limiter<<<dim3((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, g.lp), dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.x0, (const R*)t.psi, h,
-                                                (const R*)t.fns[0], (const R*)t.fns[1],
-                                                (const R*)t.fns[2], (const R*)t.cp,
+    k_adj_limiter<<<dim3((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, (g.lp + FKB - 1) / FKB), dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.x0, (const R*)t.psi, h,
+                                                (const R*)t.v[0], (const R*)t.v[1],
+                                                (const R*)t.v[2], (const R*)t.cp,
                                                 (const R*)t.cn, a);
     k_adj_pseudo<<<dim3((g.np + PBX - 1) / PBX, (g.mp + PBY - 1) / PBY, (g.lp + PKB - 1) / PKB),
                    dim3(PBX, PBY, 1), 0,

@@ -29,9 +29,7 @@ struct TapeSlot {
     void* xb = nullptr;     // x with the inflow halo (upwind input)
     void* x0 = nullptr;     // x with the edge-copied halo (limiter bounds)
     void* psi = nullptr;    // upwind result with the mirrored halo
-    void* v[3] = {nullptr, nullptr, nullptr};
-    void* fns[3] = {nullptr, nullptr, nullptr};
-    void* vl[3] = {nullptr, nullptr, nullptr};
+    void* v[3] = {nullptr, nullptr, nullptr};  // pseudo-velocities
     void* cp = nullptr;
     void* cn = nullptr;
 };

@@ -75,17 +75,14 @@ struct FusedStd {
 // Intermediates of a standard limited step that the adjoint step reads: x
 // with the inflow values on its wall halo (xb, the upwind's input) and with
 // edge copies there (x0, the limiter's bounds), the upwind result xant (psi)
-// with its mirrored x/y halo ring, the pseudo-velocities V, their donor
-// fluxes w (the limiter's inputs) and the limited velocities vl on the x-,
-// y- and z-faces, and cp, cn.
+// with its mirrored x/y halo ring, the pseudo-velocities V on the x-, y- and
+// z-faces, and cp, cn.
 template <typename T>
 struct StdTape {
     T* xb = nullptr;
     T* x0 = nullptr;
     T* psi = nullptr;
     T* v[3] = {nullptr, nullptr, nullptr};
-    T* w[3] = {nullptr, nullptr, nullptr};
-    T* vl[3] = {nullptr, nullptr, nullptr};
     T* cp = nullptr;
     T* cn = nullptr;
 };

@@ -847,7 +847,7 @@ long long items_of(const Geo& g, int clen) {
 // (two planes of xant and one of V/cp/cn recomputed below the chunk, two
 // above it, and the item's set-up).
 constexpr int kRamp = 5;
-long long step_cost(long long ntile, int lp, int clen, long long per_xcd) {
+[[maybe_unused]] long long step_cost(long long ntile, int lp, int clen, long long per_xcd) {
     const long long nchunk = (lp + clen - 1) / clen;
     const long long tiles_per_xcd = (ntile + NXCD - 1) / NXCD;
     const long long rounds = (tiles_per_xcd * nchunk + per_xcd - 1) / per_xcd;

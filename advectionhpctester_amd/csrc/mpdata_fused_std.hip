@@ -80,10 +80,9 @@ __device__ __forceinline__ T slim(T v, T a1, T a2, T xm, T xc) {
 // TAPE (limiter on, open boundaries, ih = 1): the step also writes the
 // intermediates the adjoint step reads to tp, as the per-stage recording
 // leaves them: x with inflow values / edge copies on the wall halo, xant
-// with its mirrored x/y halo ring, V, the donor fluxes of V and the limited
-// velocities on every face of the domain (zero on the x/y walls, mirrored on
-// the lids), cp and cn.  Each cell and face is written by the one position
-// that owns it.
+// with its mirrored x/y halo ring, V on every face of the domain (zero on
+// the x/y walls, mirrored on the lids), cp and cn.  Each cell and face is
+// written by the one position that owns it.
 template <typename T, int RX, int RY, bool NONOS, bool PER, bool TAPE = false>
 __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int ntx, int tile,
                                                int k0, int k1, const T* __restrict__ x,
@@ -225,7 +224,6 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
     T v3q[2] = {T(0), T(0)}, w3q[2] = {T(0), T(0)};
     T xmx[2] = {T(0), T(0)}, xmn[2] = {T(0), T(0)};
     T f3c = T(0);  // flux through the upper z-face of the last output plane
-    T vl3c = T(0); // its limited velocity (TAPE)
 
     sx[0][pj + 1][pi + 1] = valid ? xq[1] : ld(own, Ks);
     if (t < NRING) sx[0][rsj][rsi] = ld(ring, Ks);
@@ -329,29 +327,25 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
         //      V of Q from the previous step, cp/cn of Q and Q-1; formed
         //      before B so that V of Q is dead before V of P is formed ------
         if (Dq && do_f) {
-            T F1 = T(0), F2 = T(0), vl1 = T(0), vl2 = T(0);
+            T F1 = T(0), F2 = T(0);
             if (!wall1) {
-                vl1 = v1Q;
+                T vl = v1Q;
                 if constexpr (NONOS)
-                    vl1 = slim(vl1, min3(T(1), CP(2, li, lj), CN(2, li - 1, lj)),
-                               min3(T(1), CP(2, li - 1, lj), CN(2, li, lj)), XA(2, li - 1, lj),
-                               XA(2, li, lj));
-                F1 = donor(RC(2, li - 1, lj), RC(2, li, lj), vl1);
+                    vl = slim(vl, min3(T(1), CP(2, li, lj), CN(2, li - 1, lj)),
+                              min3(T(1), CP(2, li - 1, lj), CN(2, li, lj)), XA(2, li - 1, lj),
+                              XA(2, li, lj));
+                F1 = donor(RC(2, li - 1, lj), RC(2, li, lj), vl);
             }
             if (!wall2) {
-                vl2 = v2Q;
+                T vl = v2Q;
                 if constexpr (NONOS)
-                    vl2 = slim(vl2, min3(T(1), CP(2, li, lj), CN(2, li, lj - 1)),
-                               min3(T(1), CP(2, li, lj - 1), CN(2, li, lj)), XA(2, li, lj - 1),
-                               XA(2, li, lj));
-                F2 = donor(RC(2, li, lj - 1), RC(2, li, lj), vl2);
+                    vl = slim(vl, min3(T(1), CP(2, li, lj), CN(2, li, lj - 1)),
+                              min3(T(1), CP(2, li, lj - 1), CN(2, li, lj)), XA(2, li, lj - 1),
+                              XA(2, li, lj));
+                F2 = donor(RC(2, li, lj - 1), RC(2, li, lj), vl);
             }
             L.sf1[pj][pi] = F1;
             L.sf2[pj][pi] = F2;
-            if constexpr (TAPE) {
-                if (own_fx) sto(pl(tp->vl[0], FXY, Q), tfo, vl1);
-                if (own_fy) sto(pl(tp->vl[1], YXY, Q), tso, vl2);
-            }
         }
         if constexpr (TAPE)
             if (Dq && own_s) sto(pl(tp->psi, SXY, Q), tso, XA(2, li, lj));
@@ -449,14 +443,8 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
             if constexpr (NONOS) L.sw2[pj][pi] = w2;
             if constexpr (TAPE) {
                 if (P >= k0 && P <= k1) {
-                    if (own_fx) {
-                        sto(pl(tp->v[0], FXY, P), tfo, val1);
-                        sto(pl(tp->w[0], FXY, P), tfo, w1);
-                    }
-                    if (own_fy) {
-                        sto(pl(tp->v[1], YXY, P), tso, val2);
-                        sto(pl(tp->w[1], YXY, P), tso, w2);
-                    }
+                    if (own_fx) sto(pl(tp->v[0], FXY, P), tfo, val1);
+                    if (own_fy) sto(pl(tp->v[1], YXY, P), tso, val2);
                 }
             }
         }
@@ -516,21 +504,16 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
         }
         // limited flux through face Q+1 of the own column (cp/cn of Q from
         // the previous step, of Q+1 from this one)
-        T vlu = T(0);  // the limited velocity through that face (TAPE)
         auto flux_up = [&]() -> T {
             T vl = v3q2;
             if constexpr (NONOS)
                 vl = slim(vl, min3(T(1), CP(1, li, lj), CN(2, li, lj)),
                           min3(T(1), CP(2, li, lj), CN(1, li, lj)), XA(2, li, lj), XA(1, li, lj));
-            vlu = vl;
             return donor(RC(2, li, lj), RC(1, li, lj), vl);
         };
         // the lower z-face flux of the first output plane of the range is
         // the upper face of plane k0-1, formed on the step before
-        if (Q == k0 - 1 && (pz || Q >= 1) && do_d) {
-            f3c = flux_up();
-            vl3c = vlu;
-        }
+        if (Q == k0 - 1 && (pz || Q >= 1) && do_d) f3c = flux_up();
         // ---- D: the new x on plane Q (antidiff3d-standard-gpubc.f90:431-459)
         if (D_on) {
             const bool gnd = !pz && Q == 1, sky = !pz && Q == lp;
@@ -549,29 +532,14 @@ __device__ __forceinline__ void fused_std_item(const Geo& g, const Off& o, int n
             }
             f3c = F3p;
             if constexpr (TAPE) {
-                // z-faces: the lower face Q of the cell (face 1 = -face 2 on
-                // the ground), and face lp+1 = -face lp on the sky plane;
+                // V3 on the lower face Q of the cell (face 1 = -face 2 on
+                // the ground), and on face lp+1 = -face lp on the sky plane;
                 // cp and cn of the cell
                 T* const tv = pl(tp->v[2], SXY, Q);
-                T* const tw = pl(tp->w[2], SXY, Q);
-                T* const tl = pl(tp->vl[2], SXY, Q);
-                if (gnd) {
-                    sto(tv, so, -v3q2);
-                    sto(tw, so, -w3q2);
-                    sto(tl, so, -vlu);
-                } else {
-                    sto(tv, so, v3q1);
-                    sto(tw, so, w3q1);
-                    sto(tl, so, vl3c);
-                }
-                if (sky) {
-                    sto(tv + SXY, so, -v3q1);
-                    sto(tw + SXY, so, -w3q1);
-                    sto(tl + SXY, so, -vl3c);
-                }
+                sto(tv, so, gnd ? -v3q2 : v3q1);
+                if (sky) sto(tv + SXY, so, -v3q1);
                 sto(pl(tp->cp, SXY, Q), so, CP(2, li, lj));
                 sto(pl(tp->cn, SXY, Q), so, CN(2, li, lj));
-                vl3c = vlu;
                 // x with the inflow values on the wall halo (the upwind's
                 // input) and with edge copies there (the limiter's bounds)
                 const T xc = ldo(pl(x, SXY, Q), so);
@@ -668,6 +636,45 @@ hipError_t launch_std(const Geo& g, const Off& o, int clen, const T* x, T* xout,
     return hipGetLastError();
 }
 
+// Tiling and k-chunk of the standard kernel: 32x16 tiles (the 32x32 build
+// spills registers and measured slower at every size), and the chunk length
+// of least modelled wall time, rounds x (clen + 7): the workgroup slots of
+// the device take the work items in rounds, and an item marches its clen
+// output planes plus the planes recomputed below and above it and its
+// set-up (measured: 128x128x64 78 us at 8 planes against 89 at 16;
+// 64x64x32 4 planes; 256x256x128 and 512x512x256 64 planes).
+// MPDATA_FUSED_CFG / _CHUNK override both for tests and tuning.
+template <typename K>
+void choose_std(const Geo& g, K kernel, int& cfg, int& clen) {
+    const int env_cfg = fused_config(), env_clen = fused_chunk_env();
+    cfg = env_cfg >= 0 ? env_cfg : 1;
+    if (env_clen > 0) {
+        clen = env_clen;
+    } else {
+        int dev = 0, ncu = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            ncu < 1)
+            ncu = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 32 * 16, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        (void)hipGetLastError();
+        const long long slots = (long long)ncu * per_cu;
+        long long best = -1;
+        clen = 64;
+        for (int c : {64, 32, 16, 8, 4}) {
+            const long long rounds = (items_of<32, 16>(g, c) + slots - 1) / slots;
+            const long long cost = rounds * (std::min(c, g.lp) + 7);
+            if (best < 0 || cost < best) {
+                best = cost;
+                clen = c;
+            }
+        }
+    }
+    if (clen > g.lp) clen = g.lp;
+}
+
 template <typename T, int RX, int RY, int MINB>
 hipError_t launch_std_any(const Geo& g, const Off& o, int clen, int nonos, const T* x, T* xout,
                           const T* u1, const T* u2, const T* u3, const T* rhr, const T* h,
@@ -688,8 +695,8 @@ hipError_t FusedStd<T>::step(const Geo& g, int nonos, const T* x, T* xout, const
                              const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
                              const T* bcy, hipStream_t s) {
     const Off o = make_off(g);
-    int cfg = 0, cl = 64;
-    choose_tiling<T>(g, cfg, cl);
+    int cfg = 1, cl = 64;
+    choose_std(g, k_fused_std<T, 32, 16, 4, true, false>, cfg, cl);
     hipError_t e = cfg == 0
                        ? launch_std_any<T, 32, 32, 4>(g, o, cl, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s)
                        : launch_std_any<T, 32, 16, 4>(g, o, cl, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s);
@@ -704,8 +711,8 @@ hipError_t FusedStd<T>::record(const Geo& g, const T* x, T* xout, const T* u1, c
                                const StdTape<T>& tp, hipStream_t s) {
     if (g.px || g.py || g.pz || g.ih != 1) return hipErrorInvalidValue;
     const Off o = make_off(g);
-    int cfg = 0, cl = 64;
-    choose_tiling<T>(g, cfg, cl);
+    int cfg = 1, cl = 64;
+    choose_std(g, k_fused_std<T, 32, 16, 2, true, false, true>, cfg, cl);
     // xout: the interior and the edge copies; the other halo cells are left
     // alone (the recorded step defines only the interior)
     return launch_std<T, 32, 16, 2, true, false, true>(g, o, cl, x, xout, u1, u2, u3, rhr, h, bcx,
