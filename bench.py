@@ -29,9 +29,10 @@ Prints one JSON line:
 Sub-objects: unfused_stages (the per-stage kernels), ensemble (64 uneven
 tracers in one persistent launch / HIP graph), multifield (eight fields on one
 grid), density (the density step with the velocity predictor), configs (the
-periodic-standard and deformational-flow configurations, per-stage kernels),
-adjoint (forward+backward against one forward).  --density-only and
---configs-only print the density or configs object alone.
+periodic-standard and deformational-flow configurations, fused kernels),
+adjoint (forward+backward against one forward).  --density-only,
+--configs-only, --multifield-only and --adjoint-only print that object
+alone.
 """
 import argparse
 import json
@@ -75,8 +76,12 @@ def parse():
                     help="run only the periodic-standard and deformational-flow configurations "
                          "(--steps steps each) and print their JSON object")
     ap.add_argument("--no-configs", action="store_true",
-                    help="skip the periodic-standard and deformational-flow configurations "
-                         "(per-stage kernels)")
+                    help="skip the periodic-standard and deformational-flow configurations")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint cost measurement and print its JSON object")
+    ap.add_argument("--multifield-only", action="store_true",
+                    help="run only the many-field measurement (--steps steps) and print its "
+                         "JSON object")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint cost measurement (128x128x64, 50 steps)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -116,6 +121,12 @@ def main():
         return
     if a.configs_only:
         print(json.dumps(configs_bench(a.steps)), flush=True)
+        return
+    if a.adjoint_only:
+        print(json.dumps(adjoint_bench()), flush=True)
+        return
+    if a.multifield_only:
+        print(json.dumps(multifield_bench(a.steps)), flush=True)
         return
     import torch
 
@@ -406,7 +417,8 @@ def multifield_bench(steps, size=(256, 256, 128), nfields=8):
             "gauge_one_by_one_ms_per_step": ms_single,
             "speedup_together_vs_one_by_one": ms_single / ms_together,
             "ce_mix_in_place_ms_per_step": ms_mixed,
-            "ce_mix": "alg_choice 0: th, qv, qc standard (stage kernels), u, v, w, tht, pstr gauge"}
+            "ce_mix": "alg_choice 0: th, qv, qc standard (fused standard kernel, one at a time), "
+                      "u, v, w, tht, pstr gauge (one batched launch)"}
 
 
 # Arrays one call of the density step with a fixed time step must move per
@@ -421,8 +433,10 @@ def configs_bench(steps):
     """The periodic-standard and deformational-flow configurations.
       periodic: 64x64x32, periodic in all directions, standard scheme, 2
         iterations, limiter off, solid-body rotation about z at Courant 0.5,
-        Gaussian sphere (testdata.periodic_rotation_case); 100 steps; per-stage
-        kernels (periodic walls are outside the fused kernel); 56 B/cell.
+        Gaussian sphere (testdata.periodic_rotation_case); the reference's
+        time loop entry (mpdata_c_timeloop) over 100 steps on the fused
+        standard kernel; stages_ms_per_step the same steps on the per-stage
+        kernels; 56 B/cell.
       deformational: 256x256x128, time-varying deformational flow at Courant
         0.7 with the velocity predictor every step, gauge, 3 iterations,
         limiter on, slotted cylinders, FP64 and FP32.  A step is the one-pass
@@ -469,10 +483,11 @@ def configs_bench(steps):
     t = {k: torch.from_numpy(np.ascontiguousarray(v)).to(dev) for k, v in c.items()}
     with mpdata.Domain(*dims, ibc=(1, 1, 1)) as d:
         def periodic(n):
-            for _ in range(n):
-                d.step(t["x"], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"],
+            d.timeloop(n, t["x"], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"],
                        variant=mpdata.STANDARD, iters=2, nonos=0)
         ms = timed(periodic, 100)
+        d.set_path(mpdata.PATH_STAGES)
+        ms_stages = timed(periodic, 100)
     orc = Oracle(np.float64)
     x = c["x"].copy()
     n_cpu = 100
@@ -482,8 +497,8 @@ def configs_bench(steps):
                  (1, 1, 1), dims + (1,))
     dt_cpu = time.perf_counter() - t0
     out["periodic_64x64x32_standard"] = {
-        "ms_per_step": ms, "value": cells1 / (ms / 1e3), "path": "stages",
-        "roofline": roof(cells1, ms, 56),
+        "ms_per_step": ms, "value": cells1 / (ms / 1e3), "path": "fused",
+        "stages_ms_per_step": ms_stages, "roofline": roof(cells1, ms, 56),
         "cpu_baseline": {"value": cells1 * n_cpu / dt_cpu, "unit": "cell-updates/s", "cores": 1,
                          "kind": "port",
                          "sample": f"{n_cpu} steps of the same 64x64x32 workload with "

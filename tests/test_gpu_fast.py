@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 import torch
 
-from gpu_helpers import interior, to_dev, to_host
+from gpu_helpers import bits_equal, interior, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
@@ -103,9 +103,11 @@ def test_fast_path_fp32_against_fp64_oracle(mp, orc64):
     assert err < 1e-5
 
 
-def test_fast_path_scope(mp):
-    """Options outside the fused kernel's scope are refused, not run
-    elsewhere."""
+def test_fast_path_scope(mp, orc64):
+    """Options outside the fused kernels' scope are refused, not run
+    elsewhere; the standard variant, which has no fast build, runs its
+    exactly rounded fused kernel."""
+    from advectionhpctester_amd.testdata import random_case
     dims = (24, 20, 10, 1)
     with mp.Domain(*dims) as d:
         d.set_path(mp.PATH_FAST)
@@ -113,5 +115,15 @@ def test_fast_path_scope(mp):
         h = d.zeros(0) + 1
         bcx = torch.zeros((2, 10, 20), dtype=torch.float64, device="cuda:0")
         bcy = torch.zeros((2, 10, 24), dtype=torch.float64, device="cuda:0")
-        with pytest.raises(mp.MpdataError):
-            d.step(z["x"], z["u1"], z["u2"], z["u3"], h, h, bcx, bcy, variant=mp.STANDARD)
+        for variant, iters, nonos in ((mp.GAUGE, 2, 0), (mp.STANDARD, 3, 1)):
+            with pytest.raises(mp.MpdataError):
+                d.step(z["x"], z["u1"], z["u2"], z["u3"], h, h, bcx, bcy, variant=variant,
+                       iters=iters, nonos=nonos)
+        c = random_case(*dims, seed=3)
+        t = {k: to_dev(v) for k, v in c.items()}
+        d.step(t["x"], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], t["bcx"], t["bcy"],
+               variant=mp.STANDARD)
+        xo = c["x"].copy()
+        orc64.step(mp.STANDARD, 2, 1, xo, c["u1"], c["u2"], c["u3"], c["rhr"], c["h"], c["bcx"],
+                   c["bcy"], (0, 0, 0), dims)
+        assert bits_equal(interior(to_host(t["x"])), interior(xo))
