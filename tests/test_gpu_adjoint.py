@@ -190,6 +190,39 @@ def test_dot_product(mpd, nsteps):
     assert rel <= 1e-12
 
 
+def test_dot_product_full_size(mpd):
+    """The dot-product test on the adjoint workload's grid (128x128x64, the
+    deformational flow and field of the gradient problem, 3 steps): the
+    tangent-linear and adjoint steps agree over 1e6 cells and 3.2e6 faces."""
+    from advectionhpctester_amd import testdata
+    c = testdata.adjoint_case(128, 128, 64, nsteps=3)
+    f = c.pop("factors")
+    c.pop("obs")
+    for k in ("u1", "u2", "u3"):
+        c[k] = c[k] * f[0]
+    nsteps = 3
+    d = {k: to_dev(v) for k, v in c.items()}
+    rng = np.random.default_rng(17)
+    dxn = _zero_halo(rng.standard_normal(c["x"].shape))
+    dun = [rng.standard_normal(c[k].shape) for k in ("u1", "u2", "u3")]
+    ybar = _zero_halo(rng.standard_normal(c["x"].shape))
+    with mpd.Domain(128, 128, 64) as dom:
+        x, dx = d["x"].clone(), to_dev(dxn)
+        du = [to_dev(v) for v in dun]
+        print("dot product at 128x128x64, 3 steps", flush=True)
+        for _ in range(nsteps):
+            dom.tl_step(x, dx, d["u1"], d["u2"], d["u3"], *du, d["rhr"], d["h"], d["bcx"],
+                        d["bcy"], variant=mpd.STANDARD, iters=2, nonos=1)
+        lhs = float((to_host(dx)[I] * ybar[I]).sum())
+        grads, _ = _adjoint_chain(mpd, dom, d, nsteps, ybar)
+    rhs = float((dxn[I] * grads[0][I]).sum()) + sum(float((a * b).sum())
+                                                    for a, b in zip(dun, grads[1:]))
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"dot product at 128x128x64: <TL,y>={lhs:.15e} <x,AD>={rhs:.15e} rel {rel:.1e}",
+          flush=True)
+    assert rel <= 1e-12
+
+
 def _torch_J(c, nsteps, factors, obs):
     """J = 0.5 |x_T - obs|^2 through the CPU reference and its autograd
     gradient with respect to (x0, u1, u2, u3)."""
