@@ -89,6 +89,7 @@ def _load():
     lib.mpdata_tl_step_f64.argtypes = [_P, _I, _I, _I] + [_P] * 12 + [_P]
     lib.mpdata_adj_reserve.argtypes = [_P, _I]
     lib.mpdata_adj_record_f64.argtypes = [_P, _I] + [_P] * 9 + [_P]
+    lib.mpdata_adj_record_variant_f64.argtypes = [_P, _I, _I] + [_P] * 9 + [_P]
     lib.mpdata_adj_step_f64.argtypes = [_P, _I] + [_P] * 9 + [_D, _P]
     lib.mpdata_dev_alloc.argtypes = [_LL, ctypes.POINTER(_P)]
     lib.mpdata_dev_free.argtypes = [_P]

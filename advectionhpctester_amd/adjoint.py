@@ -1,6 +1,6 @@
 """Gradients through many MPDATA steps: a torch.autograd.Function over the
-hand-written adjoint kernels of libmpdata_hip (standard non-oscillatory
-scheme, 2 iterations, open walls, FP64).
+hand-written adjoint kernels of libmpdata_hip (non-oscillatory scheme,
+standard or gauge variant, 2 iterations, open walls, FP64).
 
 Forward runs ``nsteps`` steps and keeps the field every ``ckpt`` steps; the
 steps of the last segment run as recorded steps, which keep their
@@ -49,7 +49,7 @@ class MpdataSteps(torch.autograd.Function):
     interiors; halo cells of x carry no gradient)."""
 
     @staticmethod
-    def forward(ctx, x0, u1, u2, u3, dom, rhr, h, bcx, bcy, nsteps, ckpt, factors):
+    def forward(ctx, x0, u1, u2, u3, dom, rhr, h, bcx, bcy, nsteps, ckpt, factors, variant):
         factors = [1.0] * nsteps if factors is None else [float(f) for f in factors]
         if len(factors) != nsteps:
             raise ValueError("one velocity factor per step")
@@ -70,16 +70,17 @@ class MpdataSteps(torch.autograd.Function):
             if taped and n >= last0:
                 if spare is None:
                     spare = torch.empty_like(x)
-                dom.adj_record(n - last0, x, spare, *un, rhr, h, bcx, bcy)
+                dom.adj_record(n - last0, x, spare, *un, rhr, h, bcx, bcy, variant=variant)
                 x, spare = spare, x
                 last_vel.append(un)
             else:
-                dom.step(x, *un, rhr, h, bcx, bcy, variant=mpdata.STANDARD, iters=2, nonos=1)
+                dom.step(x, *un, rhr, h, bcx, bcy, variant=variant, iters=2, nonos=1)
         if taped and nsteps > 0:
             _complete_halo(x, spare, x0.detach())
         ctx.tape_epoch = dom.tape_epoch if taped else None
         ctx.last_vel = last_vel
         ctx.dom, ctx.nsteps, ctx.ckpt, ctx.factors = dom, nsteps, ckpt, factors
+        ctx.variant = variant
         ctx.ckpts = ckpts
         ctx.fields = (u1.detach(), u2.detach(), u3.detach(), rhr, h, bcx, bcy)
         return x
@@ -107,17 +108,23 @@ class MpdataSteps(torch.autograd.Function):
                 vel = [[_scaled(u, factors[n]) for u in (u1, u2, u3)] for n in range(s0, s1)]
                 for n in range(s0, s1):
                     out = bufs[(n - s0) & 1]
-                    dom.adj_record(n - s0, x, out, *vel[n - s0], rhr, h, bcx, bcy)
+                    dom.adj_record(n - s0, x, out, *vel[n - s0], rhr, h, bcx, bcy,
+                                   variant=ctx.variant)
                     x = out
             for n in reversed(range(s0, s1)):
                 # u_n = f_n u: the gradient of u collects f_n times that of u_n
                 dom.adj_step(n - s0, *vel[n - s0], rhr, h, xbar, *ub, ufac=factors[n])
         ctx.ckpts = None
         ctx.last_vel = None
-        return (xbar, ub[0], ub[1], ub[2]) + (None,) * 8
+        return (xbar, ub[0], ub[1], ub[2]) + (None,) * 9
 
 
-def advect(x0, u1, u2, u3, dom, rhr, h, bcx, bcy, nsteps, ckpt=10, factors=None):
-    """nsteps standard non-oscillatory MPDATA steps, differentiable in x0 and
-    the face velocities, with checkpoints every `ckpt` steps."""
-    return MpdataSteps.apply(x0, u1, u2, u3, dom, rhr, h, bcx, bcy, nsteps, ckpt, factors)
+def advect(x0, u1, u2, u3, dom, rhr, h, bcx, bcy, nsteps, ckpt=10, factors=None,
+           variant=mpdata.STANDARD):
+    """nsteps non-oscillatory MPDATA steps (standard or gauge variant, 2
+    iterations), differentiable in x0 and the face velocities, with
+    checkpoints every `ckpt` steps."""
+    if variant not in (mpdata.STANDARD, mpdata.GAUGE):
+        raise ValueError("the adjoint covers the standard and gauge variants")
+    return MpdataSteps.apply(x0, u1, u2, u3, dom, rhr, h, bcx, bcy, nsteps, ckpt, factors,
+                             variant)

@@ -195,6 +195,33 @@ __device__ __forceinline__ void adj_face_flux(R* L, R fb, R v, R pm, R pc, R rm,
     }
 }
 
+// ---- gauge limited fluxes (antidiff3d-gauge-gpubc.f90:405-422), one face --
+// F = pp(V)*min3(1, cp_c, cn_m) - pn(V)*min3(1, cp_m, cn_c)
+__device__ __forceinline__ void adj_face_gflux(R* L, R fb, R v, R cp_c, R cn_m, R cp_m, R cn_c,
+                                               int cpb_c, int cnb_m, int cpb_m, int cnb_c,
+                                               R* vbar) {
+    const R m1 = R(1) < cp_c ? R(1) : cp_c;
+    const R a1 = m1 < cn_m ? m1 : cn_m;
+    const R m2 = R(1) < cp_m ? R(1) : cp_m;
+    const R a2 = m2 < cn_c ? m2 : cn_c;
+    *vbar = fb * (ind_pos(v) * a1 + ind_neg(v) * a2);
+    const R a1b = fb * (v > R(0) ? v : R(0));
+    const R a2b = -(fb * (v < R(0) ? -v : R(0)));
+    if (m1 < cn_m) {
+        if (!(R(1) < cp_c)) atomicAdd(&L[cpb_c], a1b);
+    } else {
+        atomicAdd(&L[cnb_m], a1b);
+    }
+    if (m2 < cn_c) {
+        if (!(R(1) < cp_m)) atomicAdd(&L[cpb_m], a2b);
+    } else {
+        atomicAdd(&L[cnb_c], a2b);
+    }
+}
+
+// GAUGE: the gauge variant's fluxes (no xant in the flux), else the
+// standard variant's donor fluxes of the limited velocity.
+template <bool GAUGE>
 __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict__ v1,
                                                       const R* __restrict__ v2,
                                                       const R* __restrict__ v3,
@@ -221,6 +248,11 @@ __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict
         const R sc = S_(a.sum, i, j, k);
         if (i > 1) {
             const R fb = S_(a.sum, i - 1, j, k) - sc;
+            if constexpr (GAUGE)
+                adj_face_gflux(L, fb, X_(v1, i, j, k), S_(cp, i, j, k), S_(cn, i - 1, j, k),
+                               S_(cp, i - 1, j, k), S_(cn, i, j, k), rcp.at(i, j, k), rcn.at(i - 1, j, k),
+                               rcp.at(i - 1, j, k), rcn.at(i, j, k), XP(a.v1, i, j, k));
+            else
             adj_face_flux(L, fb, X_(v1, i, j, k), S_(psi, i - 1, j, k), pc,
                           S_(rhr, i - 1, j, k), rc, S_(cp, i, j, k), S_(cn, i - 1, j, k),
                           S_(cp, i - 1, j, k), S_(cn, i, j, k), rp.at(i - 1, j, k), rp.at(i, j, k),
@@ -229,6 +261,11 @@ __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict
         }
         if (j > 1) {
             const R fb = S_(a.sum, i, j - 1, k) - sc;
+            if constexpr (GAUGE)
+                adj_face_gflux(L, fb, Y_(v2, i, j, k), S_(cp, i, j, k), S_(cn, i, j - 1, k),
+                               S_(cp, i, j - 1, k), S_(cn, i, j, k), rcp.at(i, j, k), rcn.at(i, j - 1, k),
+                               rcp.at(i, j - 1, k), rcn.at(i, j, k), YP(a.v2, i, j, k));
+            else
             adj_face_flux(L, fb, Y_(v2, i, j, k), S_(psi, i, j - 1, k), pc,
                           S_(rhr, i, j - 1, k), rc, S_(cp, i, j, k), S_(cn, i, j - 1, k),
                           S_(cp, i, j - 1, k), S_(cn, i, j, k), rp.at(i, j - 1, k), rp.at(i, j, k),
@@ -239,6 +276,11 @@ __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict
             // the ground cell counts its upper face twice, the sky cell its lower
             const R c1 = k - 1 == 1 ? R(2) : R(1), c2 = sky ? R(2) : R(1);
             const R fb = c1 * S_(a.sum, i, j, k - 1) - c2 * sc;
+            if constexpr (GAUGE)
+                adj_face_gflux(L, fb, Z_(v3, i, j, k), S_(cp, i, j, k), S_(cn, i, j, k - 1),
+                               S_(cp, i, j, k - 1), S_(cn, i, j, k), rcp.at(i, j, k), rcn.at(i, j, k - 1),
+                               rcp.at(i, j, k - 1), rcn.at(i, j, k), ZP(a.v3, i, j, k));
+            else
             adj_face_flux(L, fb, Z_(v3, i, j, k), S_(psi, i, j, k - 1), pc,
                           S_(rhr, i, j, k - 1), rc, S_(cp, i, j, k), S_(cn, i, j, k - 1),
                           S_(cp, i, j, k - 1), S_(cn, i, j, k), rp.at(i, j, k - 1), rp.at(i, j, k),
@@ -271,6 +313,10 @@ __device__ __forceinline__ void pick_min(Pick& p, R v, int src, int di, int dj, 
     if (!(p.val < v)) p = Pick{v, src, di, dj, dk};   // min2(lo, t) = lo < t ? lo : t
 }
 
+// GAUGE: the limiter bounds the pseudo-velocities themselves and their
+// adjoints go to those of V; else the donor fluxes of V (formed again here)
+// and their adjoints go to a.f1..f3.
+template <bool GAUGE>
 __global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restrict__ x0,
                                                          const R* __restrict__ psi,
                                                          const R* __restrict__ h,
@@ -316,20 +362,30 @@ __global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restr
 #undef ACC
         if (!(mx.val > mxo.val)) mx = mxo;   // mx = max2(mx, mxo)
         if (!(mn.val < mno.val)) mn = mno;   // mn = min2(mn, mno)
-        // the limiter's inputs, the donor fluxes of the pseudo-velocities
-        // (:182-202), formed again from the tape's V and xant (zero V on the
-        // x/y walls gives zero fluxes there)
-        const R pc_ = S_(psi, i, j, k);
-        const R a1p = donor(pc_, S_(psi, i + 1, j, k), X_(v1, i + 1, j, k));
-        const R a1 = donor(S_(psi, i - 1, j, k), pc_, X_(v1, i, j, k));
-        const R a2p = donor(pc_, S_(psi, i, j + 1, k), Y_(v2, i, j + 1, k));
-        const R a2 = donor(S_(psi, i, j - 1, k), pc_, Y_(v2, i, j, k));
-        R a3p, a3;
-        if (gnd) { a3p = donor(pc_, S_(psi, i, j, k + 1), Z_(v3, i, j, k + 1)); a3 = -a3p; }
-        else if (sky) { a3 = donor(S_(psi, i, j, k - 1), pc_, Z_(v3, i, j, k)); a3p = -a3; }
-        else {
-            a3p = donor(pc_, S_(psi, i, j, k + 1), Z_(v3, i, j, k + 1));
-            a3 = donor(S_(psi, i, j, k - 1), pc_, Z_(v3, i, j, k));
+        // the limiter's inputs: the pseudo-velocities (gauge), or their donor
+        // fluxes (standard, :182-202), formed again from the tape's V and
+        // xant (zero V on the x/y walls gives zero fluxes there)
+        R a1p, a1, a2p, a2, a3p, a3;
+        if constexpr (GAUGE) {
+            a1p = X_(v1, i + 1, j, k);
+            a1 = X_(v1, i, j, k);
+            a2p = Y_(v2, i, j + 1, k);
+            a2 = Y_(v2, i, j, k);
+            if (gnd) { a3p = Z_(v3, i, j, k + 1); a3 = -a3p; }
+            else if (sky) { a3 = Z_(v3, i, j, k); a3p = -a3; }
+            else { a3p = Z_(v3, i, j, k + 1); a3 = Z_(v3, i, j, k); }
+        } else {
+            const R pc_ = S_(psi, i, j, k);
+            a1p = donor(pc_, S_(psi, i + 1, j, k), X_(v1, i + 1, j, k));
+            a1 = donor(S_(psi, i - 1, j, k), pc_, X_(v1, i, j, k));
+            a2p = donor(pc_, S_(psi, i, j + 1, k), Y_(v2, i, j + 1, k));
+            a2 = donor(S_(psi, i, j - 1, k), pc_, Y_(v2, i, j, k));
+            if (gnd) { a3p = donor(pc_, S_(psi, i, j, k + 1), Z_(v3, i, j, k + 1)); a3 = -a3p; }
+            else if (sky) { a3 = donor(S_(psi, i, j, k - 1), pc_, Z_(v3, i, j, k)); a3p = -a3; }
+            else {
+                a3p = donor(pc_, S_(psi, i, j, k + 1), Z_(v3, i, j, k + 1));
+                a3 = donor(S_(psi, i, j, k - 1), pc_, Z_(v3, i, j, k));
+            }
         }
         const R hc = S_(h, i, j, k);
         const R qp = (a1p < R(0) ? -a1p : R(0)) + (a1 > R(0) ? a1 : R(0)) + (a2p < R(0) ? -a2p : R(0)) +
@@ -368,16 +424,16 @@ __global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restr
     __syncthreads();
     flush_plane<SCAL>(g, L, rp, k - 1, a.psi, tid);
     flush_plane<SCAL>(g, L, rx, k - 1, a.x0, tid);
-    flush_plane<FACEX>(g, L, r1, k, a.f1, tid);
-    flush_plane<FACEY>(g, L, r2, k, a.f2, tid);
-    flush_plane<FACEZ>(g, L, r3, k, a.f3, tid);
+    flush_plane<FACEX>(g, L, r1, k, GAUGE ? a.v1 : a.f1, tid);
+    flush_plane<FACEY>(g, L, r2, k, GAUGE ? a.v2 : a.f2, tid);
+    flush_plane<FACEZ>(g, L, r3, k, GAUGE ? a.v3 : a.f3, tid);
     __syncthreads();
     }
     flush_plane<SCAL>(g, L, rp, kb1, a.psi, tid);
     flush_plane<SCAL>(g, L, rp, kb1 + 1, a.psi, tid);
     flush_plane<SCAL>(g, L, rx, kb1, a.x0, tid);
     flush_plane<SCAL>(g, L, rx, kb1 + 1, a.x0, tid);
-    flush_plane<FACEZ>(g, L, r3, kb1 + 1, a.f3, tid);
+    flush_plane<FACEZ>(g, L, r3, kb1 + 1, GAUGE ? a.v3 : a.f3, tid);
 }
 
 // ---- pseudo-velocities and first-order fluxes (:72-202) --------------------
@@ -443,6 +499,41 @@ __device__ __forceinline__ void adj_face_pseudo(R* L, R vb, R fb, R V, R x1, R x
     atomicAdd(&L[as], uf * ab);
 }
 
+// gauge corrective term -((0.125*a)*r)*cr, cr = sum of b * rat4 over the
+// corr terms; this corr's share (y1 = its rat4 value, plus pair 0,1 and
+// minus pair 2,3 with weights +-0.25)
+__device__ __forceinline__ void adj_gcorr(R* L, R vb, R a, const Corr& c, R r, R& ab, R uf) {
+    const R crb = -(vb * (R(0.125) * a * r));
+    const R bb = uf * (crb * c.y1);
+#pragma unroll
+    for (int q2 = 0; q2 < 4; q2++) atomicAdd(&L[c.bf[q2]], bb);
+    const R cw = R(0.25) * c.b * crb;
+    atomicAdd(&L[c.cb[0]], cw);
+    atomicAdd(&L[c.cb[1]], cw);
+    atomicAdd(&L[c.cb[2]], -cw);
+    atomicAdd(&L[c.cb[3]], -cw);
+    ab -= vb * (R(0.125) * r) * (c.b * c.y1);
+}
+
+// gauge pseudo-velocity (antidiff3d-gauge-gpubc.f90:89-203):
+// V = (|a| - a*a*r) * ((x2 - x1)*0.5) - ((0.125*a)*r)*cr; vb its adjoint
+__device__ __forceinline__ void adj_face_gauge(R* L, R vb, R x1, R x2, R a, R r, int x1s, int x2s,
+                                               int as, const Corr& c0, const Corr& c1, bool two,
+                                               R uf) {
+    const R P = fabs(a) - a * a * r, D = (x2 - x1) * R(0.5);
+    const R xb = R(0.5) * P * vb;
+    R ab = vb * D * (sgn(a) - r * (a + a));
+    adj_gcorr(L, vb, a, c0, r, ab, uf);
+    if (two) adj_gcorr(L, vb, a, c1, r, ab, uf);
+    atomicAdd(&L[x1s], -xb);
+    atomicAdd(&L[x2s], xb);
+    atomicAdd(&L[as], uf * ab);
+}
+
+// GAUGE: the gauge variant's pseudo-velocities, whose adjoint a.v already
+// holds the limiter's part; else the standard variant's, with the adjoint
+// a.f of their donor fluxes.
+template <bool GAUGE>
 __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __restrict__ u1,
                                                           const R* __restrict__ u2,
                                                           const R* __restrict__ u3,
@@ -525,12 +616,16 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
             c.b = F1 + F2 + F3 + F4;                                                               \
             const R v0_ = AXV(I0, J0, K0), v1_ = AXV(I1, J1, K1);                                  \
             const R v2_ = AXV(I2, J2, K2), v3_ = AXV(I3, J3, K3);                                  \
-            c.sg[0] = sgn(v0_); c.sg[1] = sgn(v1_); c.sg[2] = sgn(v2_); c.sg[3] = sgn(v3_);        \
             c.cb[0] = LPS(I0, J0, K0); c.cb[1] = LPS(I1, J1, K1);                                  \
             c.cb[2] = LPS(I2, J2, K2); c.cb[3] = LPS(I3, J3, K3);                                  \
-            c.y1 = (fabs(v0_) + fabs(v1_)) - (fabs(v2_) + fabs(v3_));                              \
-            c.y2 = fabs(v0_) + fabs(v1_) + fabs(v2_) + fabs(v3_) + ep;                             \
-            c.iy2 = R(1) / c.y2;                                                                   \
+            if constexpr (GAUGE) {                                                                 \
+                c.y1 = ((v1_ + v0_) - (v3_ + v2_)) * R(0.25);                                      \
+            } else {                                                                               \
+                c.sg[0] = sgn(v0_); c.sg[1] = sgn(v1_); c.sg[2] = sgn(v2_); c.sg[3] = sgn(v3_);    \
+                c.y1 = (fabs(v0_) + fabs(v1_)) - (fabs(v2_) + fabs(v3_));                          \
+                c.y2 = fabs(v0_) + fabs(v1_) + fabs(v2_) + fabs(v3_) + ep;                         \
+                c.iy2 = R(1) / c.y2;                                                               \
+            }                                                                                      \
         }
         if (valid && i > 1) {
             const R hm = R(1) / (R(0.5) * (S_(h, i - 1, j, k) + S_(h, i, j, k)));
@@ -542,6 +637,10 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
                 CORR(c1, Z_(u3, i - 1, j, k), Z_(u3, i - 1, j, k + 1), Z_(u3, i, j, k + 1), Z_(u3, i, j, k),
                      LU3(i - 1, j, k), LU3(i - 1, j, k + 1), LU3(i, j, k + 1), LU3(i, j, k),
                      i - 1, j, k + 1, i, j, k + 1, i - 1, j, k - 1, i, j, k - 1)
+            if constexpr (GAUGE)
+                adj_face_gauge(L, X_(a.v1, i, j, k), AXV(i - 1, j, k), AXV(i, j, k), X_(u1, i, j, k), hm,
+                               LPS(i - 1, j, k), LPS(i, j, k), LU1(i, j, k), c0, c1, zterms, a.uf);
+            else
             adj_face_pseudo(L, X_(a.v1, i, j, k), X_(a.f1, i, j, k), X_(v1, i, j, k), AXV(i - 1, j, k),
                             AXV(i, j, k), X_(u1, i, j, k), hm, LPS(i - 1, j, k), LPS(i, j, k),
                             LU1(i, j, k), c0, c1, zterms, a.uf);
@@ -556,6 +655,10 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
                 CORR(c1, Z_(u3, i, j - 1, k), Z_(u3, i, j, k), Z_(u3, i, j, k + 1), Z_(u3, i, j - 1, k + 1),
                      LU3(i, j - 1, k), LU3(i, j, k), LU3(i, j, k + 1), LU3(i, j - 1, k + 1),
                      i, j - 1, k + 1, i, j, k + 1, i, j - 1, k - 1, i, j, k - 1)
+            if constexpr (GAUGE)
+                adj_face_gauge(L, Y_(a.v2, i, j, k), AXV(i, j - 1, k), AXV(i, j, k), Y_(u2, i, j, k), hm,
+                               LPS(i, j - 1, k), LPS(i, j, k), LU2(i, j, k), c0, c1, zterms, a.uf);
+            else
             adj_face_pseudo(L, Y_(a.v2, i, j, k), Y_(a.f2, i, j, k), Y_(v2, i, j, k), AXV(i, j - 1, k),
                             AXV(i, j, k), Y_(u2, i, j, k), hm, LPS(i, j - 1, k), LPS(i, j, k),
                             LU2(i, j, k), c0, c1, zterms, a.uf);
@@ -569,6 +672,10 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
             CORR(c1, Y_(u2, i, j, k - 1), Y_(u2, i, j + 1, k - 1), Y_(u2, i, j + 1, k), Y_(u2, i, j, k),
                  LU2(i, j, k - 1), LU2(i, j + 1, k - 1), LU2(i, j + 1, k), LU2(i, j, k),
                  i, j + 1, k - 1, i, j + 1, k, i, j - 1, k - 1, i, j - 1, k)
+            if constexpr (GAUGE)
+                adj_face_gauge(L, Z_(a.v3, i, j, k), AXV(i, j, k - 1), AXV(i, j, k), Z_(u3, i, j, k), hm,
+                               LPS(i, j, k - 1), LPS(i, j, k), LU3(i, j, k), c0, c1, true, a.uf);
+            else
             adj_face_pseudo(L, Z_(a.v3, i, j, k), Z_(a.f3, i, j, k), Z_(v3, i, j, k), AXV(i, j, k - 1),
                             AXV(i, j, k), Z_(u3, i, j, k), hm, LPS(i, j, k - 1), LPS(i, j, k),
                             LU3(i, j, k), c0, c1, true, a.uf);
@@ -737,12 +844,15 @@ int mpdata_adj_reserve(mpdata_ctx* c, int nslots) {
     return MPDATA_OK;
 }
 
-int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x_out,
-                          const double* u1, const double* u2, const double* u3,
-                          const double* rhr, const double* h, const double* bcx,
-                          const double* bcy, void* stream) {
+int mpdata_adj_record_variant_f64(mpdata_ctx* c, int slot, int variant, const double* x_in,
+                                  double* x_out, const double* u1, const double* u2,
+                                  const double* u3, const double* rhr, const double* h,
+                                  const double* bcx, const double* bcy, void* stream) {
     int rc = check_adj(c);
     if (rc) return rc;
+    if (variant != MPDATA_STANDARD && variant != MPDATA_GAUGE)
+        return fail(MPDATA_EINVAL, "the adjoint covers the standard and gauge variants (variant %d)",
+                    variant);
     if (slot < 0 || slot >= (int)c->tape.size())
         return fail(MPDATA_EINVAL, "tape slot %d outside 0..%d (mpdata_adj_reserve)", slot,
                     (int)c->tape.size() - 1);
@@ -759,6 +869,25 @@ int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x
     R *xb = (R*)t.xb, *x0 = (R*)t.x0, *psi = (R*)t.psi, *cp = (R*)t.cp, *cn = (R*)t.cn;
     R* v[3];
     for (int d = 0; d < 3; d++) v[d] = (R*)t.v[d];
+    t.variant = variant;
+    if (variant == MPDATA_GAUGE) {
+        // the gauge step's stage sequence (antidiff3d-gauge-gpubc.f90:71-430):
+        // the limiter bounds V itself; the limited fluxes go nowhere
+        k_prep_x<<<dim3((g.np + 2 * g.ih + 63) / 64, (g.mp + 2 * g.ih + 3) / 4, g.lp + 2 * g.ih),
+                   dim3(64, 4, 1), 0, s>>>(g, x_in, bcx, bcy, xb, x0);
+        HIPCHK(hipGetLastError());
+        HIPCHK(St::upwind(g, u1, u2, u3, xb, psi, rhr, h, s));
+        HIPCHK(St::halo_xy(g, psi, St::MIRROR, 1, s));
+        HIPCHK(St::pseudo_vel(g, MPDATA_GAUGE, u1, u2, u3, psi, h, v[0], v[1], v[2], nullptr,
+                              nullptr, nullptr, s));
+        HIPCHK(St::face_bc(g, v[0], v[1], v[2], s));
+        HIPCHK(St::limiter(g, x0, psi, h, v[0], v[1], v[2], cp, cn, s));
+        HIPCHK(St::halo_xy(g, cp, St::ZERO, 1, s));
+        HIPCHK(St::halo_xy(g, cn, St::ZERO, 1, s));
+        HIPCHK(St::gauge_update(g, 1, v[0], v[1], v[2], cp, cn, psi, rhr, h, x_out, nullptr,
+                                nullptr, nullptr, s));
+        return MPDATA_OK;
+    }
     if (c->path != MPDATA_PATH_STAGES) {
         // the fused standard step, writing the tape as it goes
         StdTape<R> tp;
@@ -803,6 +932,39 @@ int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x
     return MPDATA_OK;
 }
 
+int mpdata_adj_record_f64(mpdata_ctx* c, int slot, const double* x_in, double* x_out,
+                          const double* u1, const double* u2, const double* u3,
+                          const double* rhr, const double* h, const double* bcx,
+                          const double* bcy, void* stream) {
+    return mpdata_adj_record_variant_f64(c, slot, MPDATA_STANDARD, x_in, x_out, u1, u2, u3, rhr,
+                                         h, bcx, bcy, stream);
+}
+
+}  // extern "C"
+
+namespace {
+
+// reverse of the antidiffusive iteration (limited fluxes, limiter,
+// pseudo-velocities) of the recorded step in `t`, standard or gauge variant
+template <bool GAUGE>
+void adj_reverse_antidiff(const Geo& g, const TapeSlot& t, const R* u1, const R* u2, const R* u3,
+                          const R* rhr, const R* h, R* ubar1, R* ubar2, R* ubar3, const Adj& a,
+                          hipStream_t s) {
+    const R *v0 = (const R*)t.v[0], *v1 = (const R*)t.v[1], *v2 = (const R*)t.v[2];
+    const R *cp = (const R*)t.cp, *cn = (const R*)t.cn, *psi = (const R*)t.psi;
+    const dim3 fgrid((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, (g.lp + FKB - 1) / FKB);
+    const dim3 pgrid((g.np + PBX - 1) / PBX, (g.mp + PBY - 1) / PBY, (g.lp + PKB - 1) / PKB);
+    k_adj_flux<GAUGE><<<fgrid, dim3(BX, BY, 1), 0, s>>>(g, v0, v1, v2, cp, cn, psi, rhr, a);
+    k_adj_limiter<GAUGE><<<fgrid, dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.x0, psi, h, v0, v1, v2,
+                                                            cp, cn, a);
+    k_adj_pseudo<GAUGE><<<pgrid, dim3(PBX, PBY, 1), 0, s>>>(g, u1, u2, u3, psi, h, v0, v1, v2,
+                                                             ubar1, ubar2, ubar3, a);
+}
+
+}  // namespace
+
+extern "C" {
+
 int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double* u2,
                         const double* u3, const double* rhr, const double* h, double* xbar,
                         double* ubar1, double* ubar2, double* ubar3, double ufac, void* stream) {
@@ -837,18 +999,10 @@ int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double*
             g, z, xbar, rhr, h, a);
     }
     HIPCHK(hipGetLastError());
-    k_adj_flux<<<dim3((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, (g.lp + FKB - 1) / FKB), dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.v[0], (const R*)t.v[1],
-                                             (const R*)t.v[2],
-                                             (const R*)t.cp, (const R*)t.cn, (const R*)t.psi, rhr, a);
-    k_adj_limiter<<<dim3((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, (g.lp + FKB - 1) / FKB), dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.x0, (const R*)t.psi, h,
-                                                (const R*)t.v[0], (const R*)t.v[1],
-                                                (const R*)t.v[2], (const R*)t.cp,
-                                                (const R*)t.cn, a);
-    k_adj_pseudo<<<dim3((g.np + PBX - 1) / PBX, (g.mp + PBY - 1) / PBY, (g.lp + PKB - 1) / PKB),
-                   dim3(PBX, PBY, 1), 0,
-                   s>>>(g, u1, u2, u3, (const R*)t.psi, h,
-                                               (const R*)t.v[0], (const R*)t.v[1],
-                                               (const R*)t.v[2], ubar1, ubar2, ubar3, a);
+    if (t.variant == MPDATA_GAUGE)
+        adj_reverse_antidiff<true>(g, t, u1, u2, u3, rhr, h, ubar1, ubar2, ubar3, a, s);
+    else
+        adj_reverse_antidiff<false>(g, t, u1, u2, u3, rhr, h, ubar1, ubar2, ubar3, a, s);
     k_adj_upwind<<<cgrid(g), cblock(), 0, s>>>(g, u1, u2, u3, (const R*)t.xb, rhr, h, xbar, ubar1,
                                                ubar2, ubar3, a);
     HIPCHK(hipGetLastError());

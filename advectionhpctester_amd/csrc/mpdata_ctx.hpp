@@ -32,6 +32,7 @@ struct TapeSlot {
     void* v[3] = {nullptr, nullptr, nullptr};  // pseudo-velocities
     void* cp = nullptr;
     void* cn = nullptr;
+    int variant = 2;        // MPDATA_STANDARD or MPDATA_GAUGE
 };
 
 }  // namespace mpdata

@@ -256,15 +256,16 @@ class Domain:
         # slots already reserved keep their contents
         check(LIB.mpdata_adj_reserve(self._ctx, int(nslots)))
 
-    def adj_record(self, slot, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy):
-        """One standard non-oscillatory step x_in -> x_out, keeping its
-        intermediates in tape slot `slot`."""
+    def adj_record(self, slot, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, variant=STANDARD):
+        """One non-oscillatory step (standard or gauge variant) x_in ->
+        x_out, keeping its intermediates in tape slot `slot`."""
         pu1, pu2, pu3, prhr, ph = self._fields(u1, u2, u3, rhr, h)
         pbx, pby = self._bc(bcx, bcy)
         self._tape_written()
-        check(LIB.mpdata_adj_record_f64(self._ctx, int(slot), self._p(x_in, 0, "x_in"),
-                                        self._p(x_out, 0, "x_out"), pu1, pu2, pu3, prhr, ph,
-                                        pbx, pby, self._stream()))
+        check(LIB.mpdata_adj_record_variant_f64(self._ctx, int(slot), int(variant),
+                                                self._p(x_in, 0, "x_in"),
+                                                self._p(x_out, 0, "x_out"), pu1, pu2, pu3, prhr,
+                                                ph, pbx, pby, self._stream()))
 
     def adj_step(self, slot, u1, u2, u3, rhr, h, xbar, ubar1, ubar2, ubar3, ufac=1.0):
         """xbar (adjoint of the recorded step's output) <- adjoint of its

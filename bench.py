@@ -647,7 +647,16 @@ def adjoint_bench(nsteps=50, ckpt=10, reps=3):
     non-oscillatory steps at 128x128x64 under the reversing deformational
     flow, with respect to x0 and the face velocities, checkpoints every
     `ckpt` steps: time of forward + backward against one plain forward of
-    the same steps (both include forming each step's velocity)."""
+    the same steps (both include forming each step's velocity).  The same
+    measurement over gauge steps goes under "gauge"."""
+    out = _adjoint_cost("standard", nsteps, ckpt, reps)
+    out["target_ratio"] = 4.0
+    g = _adjoint_cost("gauge", nsteps, ckpt, reps)
+    out["gauge"] = {k: g[k] for k in ("forward_ms", "forward_backward_ms", "ratio")}
+    return out
+
+
+def _adjoint_cost(variant_name, nsteps, ckpt, reps):
     import torch
 
     from advectionhpctester_amd import adjoint, mpdata, testdata
@@ -658,17 +667,19 @@ def adjoint_bench(nsteps=50, ckpt=10, reps=3):
     d = {k: torch.from_numpy(v).to(dev) for k, v in c.items()}
     rest = (d["rhr"], d["h"], d["bcx"], d["bcy"])
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    variant = {"standard": mpdata.STANDARD, "gauge": mpdata.GAUGE}[variant_name]
     with mpdata.Domain(128, 128, 64) as dom:
         def plain():
             x = d["x"].clone()
             for n in range(nsteps):
                 dom.step(x, d["u1"] * f[n], d["u2"] * f[n], d["u3"] * f[n], *rest,
-                         variant=mpdata.STANDARD, iters=2, nonos=1)
+                         variant=variant, iters=2, nonos=1)
             return x
 
         def fwd_bwd():
             leaves = [d[k].clone().requires_grad_(True) for k in ("x", "u1", "u2", "u3")]
-            xT = adjoint.advect(*leaves, dom, *rest, nsteps, ckpt=ckpt, factors=f)
+            xT = adjoint.advect(*leaves, dom, *rest, nsteps, ckpt=ckpt, factors=f,
+                                variant=variant)
             r = mpdata.interior(xT - obs)
             (0.5 * (r * r).sum()).backward()
             return leaves
@@ -689,9 +700,8 @@ def adjoint_bench(nsteps=50, ckpt=10, reps=3):
         t_fwd = timed(plain)
         t_fb = timed(fwd_bwd)
     return {"grid": [128, 128, 64], "steps": nsteps, "checkpoint_every": ckpt,
-            "variant": "standard", "iters": 2, "nonos": 1,
-            "forward_ms": t_fwd, "forward_backward_ms": t_fb, "ratio": t_fb / t_fwd,
-            "target_ratio": 4.0}
+            "variant": variant_name, "iters": 2, "nonos": 1,
+            "forward_ms": t_fwd, "forward_backward_ms": t_fb, "ratio": t_fb / t_fwd}
 
 
 def cpu_baseline(case, dims, dtype, steps):

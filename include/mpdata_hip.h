@@ -408,10 +408,11 @@ int mpdata_tl_step_f64(mpdata_ctx *ctx, int variant, int iters, int nonos, doubl
                        const double *u1, const double *u2, const double *u3, const double *du1,
                        const double *du2, const double *du3, const double *rhr, const double *h,
                        const double *bcx, const double *bcy, void *stream);
-/* Adjoint of the standard non-oscillatory 2-iteration step (open walls,
- * ih = 1, FP64 context).  mpdata_adj_reserve allocates nslots tape slots;
- * mpdata_adj_record_f64 runs one forward step x_in -> x_out (x_out as
- * mpdata_step leaves x) keeping its intermediates in `slot`;
+/* Adjoint of the non-oscillatory 2-iteration step, standard or gauge
+ * variant (open walls, ih = 1, FP64 context).  mpdata_adj_reserve allocates
+ * nslots tape slots; mpdata_adj_record_variant_f64 runs one forward step of
+ * `variant` (MPDATA_STANDARD or MPDATA_GAUGE) x_in -> x_out keeping its
+ * intermediates in `slot` (mpdata_adj_record_f64: the standard variant);
  * mpdata_adj_step_f64 maps xbar (adjoint of the interior of x_out) to the
  * adjoint of the interior of x_in, in place, and ADDS ufac times the
  * velocity adjoint to ubar1..ubar3 (zero them before the first step; ufac
@@ -422,6 +423,10 @@ int mpdata_adj_record_f64(mpdata_ctx *ctx, int slot, const double *x_in, double 
                           const double *u1, const double *u2, const double *u3,
                           const double *rhr, const double *h, const double *bcx,
                           const double *bcy, void *stream);
+int mpdata_adj_record_variant_f64(mpdata_ctx *ctx, int slot, int variant, const double *x_in,
+                                  double *x_out, const double *u1, const double *u2,
+                                  const double *u3, const double *rhr, const double *h,
+                                  const double *bcx, const double *bcy, void *stream);
 int mpdata_adj_step_f64(mpdata_ctx *ctx, int slot, const double *u1, const double *u2,
                         const double *u3, const double *rhr, const double *h, double *xbar,
                         double *ubar1, double *ubar2, double *ubar3, double ufac, void *stream);

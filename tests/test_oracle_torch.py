@@ -57,3 +57,21 @@ def test_torch_gradient_matches_central_differences():
         fd = (J(t["x"] + e * d, t["u1"]) - J(t["x"] - e * d, t["u1"])) / (2 * e)
     ad = float((x0.grad * d).sum())
     assert abs(float(fd) - ad) <= 1e-5 * abs(ad)
+
+
+@pytest.mark.parametrize("dims,seed,positive", [((9, 7, 6, 1), 11, True), ((6, 10, 5, 1), 12, False),
+                                                ((12, 12, 3, 1), 13, True)])
+def test_torch_gauge_step_equals_oracle(orc64, dims, seed, positive):
+    from oracle import GAUGE
+    c = _case(dims, seed, positive)
+    xo = c["x"].copy()
+    for _ in range(3):
+        orc64.step(GAUGE, 2, 1, xo, c["u1"], c["u2"], c["u3"], c["rhr"], c["h"], c["bcx"],
+                   c["bcy"], (0, 0, 0), dims)
+    t = {k: torch.from_numpy(v.copy()) for k, v in c.items()}
+    x = t["x"]
+    for _ in range(3):
+        x = mpdata_torch.step_gauge(x, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], t["bcx"],
+                                    t["bcy"])
+    I = (slice(1, -1),) * 3
+    assert np.array_equal(x.numpy()[I], xo[I])
