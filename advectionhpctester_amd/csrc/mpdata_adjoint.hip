@@ -870,6 +870,18 @@ int mpdata_adj_record_variant_f64(mpdata_ctx* c, int slot, int variant, const do
     R* v[3];
     for (int d = 0; d < 3; d++) v[d] = (R*)t.v[d];
     t.variant = variant;
+    StdTape<R> tp;
+    tp.xb = xb;
+    tp.x0 = x0;
+    tp.psi = psi;
+    tp.cp = cp;
+    tp.cn = cn;
+    for (int d = 0; d < 3; d++) tp.v[d] = v[d];
+    if (variant == MPDATA_GAUGE && c->path != MPDATA_PATH_STAGES) {
+        // the fused gauge step, writing the tape as it goes
+        HIPCHK(Fused<R>::record(g, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, tp, s));
+        return MPDATA_OK;
+    }
     if (variant == MPDATA_GAUGE) {
         // the gauge step's stage sequence (antidiff3d-gauge-gpubc.f90:71-430):
         // the limiter bounds V itself; the limited fluxes go nowhere
@@ -890,13 +902,6 @@ int mpdata_adj_record_variant_f64(mpdata_ctx* c, int slot, int variant, const do
     }
     if (c->path != MPDATA_PATH_STAGES) {
         // the fused standard step, writing the tape as it goes
-        StdTape<R> tp;
-        tp.xb = xb;
-        tp.x0 = x0;
-        tp.psi = psi;
-        tp.cp = cp;
-        tp.cn = cn;
-        for (int d = 0; d < 3; d++) tp.v[d] = v[d];
         HIPCHK(FusedStd<R>::record(g, x_in, x_out, u1, u2, u3, rhr, h, bcx, bcy, tp, s));
         return MPDATA_OK;
     }

@@ -81,6 +81,37 @@ hipError_t Fused<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const 
     return hipGetLastError();
 }
 
+// 32x16 tiles at 2 waves per SIMD (the tape's stores and offsets on top of
+// the step's registers) and the chunk of least modelled wall time.
+template <typename T>
+hipError_t Fused<T>::record(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                            const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+                            const StdTape<T>& tp, hipStream_t s) {
+    constexpr int RX = 32, RY = 16, MINB = 2;
+    if (g.px || g.py || g.pz || g.ih != 1) return hipErrorInvalidValue;
+    const Off o = make_off(g);
+    const int env_clen = fused_chunk_env();
+    int clen = env_clen > 0 ? env_clen : chunk_by_rounds<RX, RY>(g, k_fused_gauge_rec<T, RX, RY, MINB>);
+    if (clen > g.lp) clen = g.lp;
+    const int ntx = (g.np + RX - 5) / (RX - 4), nty = (g.mp + RY - 5) / (RY - 4);
+    const int nchunk = (g.lp + clen - 1) / clen;
+    int dev = 0, ncu = 0, per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_gauge_rec<T, RX, RY, MINB>,
+                                                         RX * RY, 0);
+    if (e != hipSuccess) return e;
+    const int grid = persistent_grid((long long)ntx * nty * nchunk,
+                                     (long long)(ntx * nty + NXCD - 1) / NXCD * nchunk, ncu, per_cu);
+    // xout: the interior and the edge copies; the other halo cells are left
+    // alone (the recorded step defines only the interior)
+    k_fused_gauge_rec<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>(g, o, ntx, nty, nchunk, clen, x,
+                                                                 u1, u2, u3, rhr, h, bcx, bcy,
+                                                                 xout, tp);
+    return hipGetLastError();
+}
+
 template struct Fused<double>;
 template struct Fused<float>;
 template struct FusedDens<double>;

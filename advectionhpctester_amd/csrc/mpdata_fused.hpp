@@ -21,6 +21,9 @@ struct PassIO {
 };
 
 template <typename T>
+struct StdTape;
+
+template <typename T>
 struct Fused {
     // True when a fused kernel implements this configuration: the gauge
     // variant with the limiter on, open walls and 2 or 3 iterations (this
@@ -36,6 +39,12 @@ struct Fused {
     static hipError_t step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
                            const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
                            hipStream_t s, int iters = 2, const FusedScratch<T>* sc = nullptr);
+    // The two-pass step with open walls and ih = 1 that also records the
+    // adjoint's tape (StdTape); xout receives the interior and the edge
+    // copies, its other halo cells are left as they were.
+    static hipError_t record(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                             const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
+                             const StdTape<T>& tp, hipStream_t s);
 };
 
 // The same fused gauge step built for speed rather than bit-identity: FMA
@@ -57,9 +66,6 @@ struct FusedFast {
 // kernels; the same layout and halo results as Fused (periodic directions'
 // halo cells keep their x values).
 template <typename T>
-struct StdTape;
-
-template <typename T>
 struct FusedStd {
     static hipError_t step(const Geo& g, int nonos, const T* x, T* xout, const T* u1,
                            const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
@@ -72,7 +78,8 @@ struct FusedStd {
                              const StdTape<T>& tp, hipStream_t s);
 };
 
-// Intermediates of a standard limited step that the adjoint step reads: x
+// Intermediates of a limited two-pass step (standard or gauge variant) that
+// the adjoint step reads: x
 // with the inflow values on its wall halo (xb, the upwind's input) and with
 // edge copies there (x0, the limiter's bounds), the upwind result xant (psi)
 // with its mirrored x/y halo ring, the pseudo-velocities V on the x-, y- and

@@ -191,7 +191,14 @@ static_assert(sizeof(FusedLds<double, 32, 32>) <= 160 * 1024, "LDS of one CU");
 // 83): stage A reads the previous pass's result pio->xa (at the mirrored
 // source column) instead of computing the upwind, u1..u3 are that pass's
 // limited velocities and x is the field at time level n (limiter bounds).
-template <typename T, int RX, int RY, bool DENS = false, bool FAST = false, int PASS = 0>
+// TAPE (the two-pass step with open walls, ih = 1): the step also writes the
+// intermediates the adjoint step reads to tp, as the per-stage recording
+// leaves them: x with inflow values / edge copies on the wall halo, xant
+// with its mirrored x/y halo ring, V on every face of the domain (zero on
+// the x/y walls, mirrored on the lids), cp and cn.  Each cell and face is
+// written by the one position that owns it.
+template <typename T, int RX, int RY, bool DENS = false, bool FAST = false, int PASS = 0,
+          bool TAPE = false>
 __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, int tile, int k0,
                                            int k1, const T* __restrict__ x, const T* __restrict__ u1,
                                            const T* __restrict__ u2, const T* __restrict__ u3,
@@ -200,7 +207,9 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                                            T* __restrict__ xo,
                                            FusedLds<T, RX, RY>& L,
                                            const DensIO<T>* dio = nullptr,
-                                           const PassIO<T>* pio = nullptr) {
+                                           const PassIO<T>* pio = nullptr,
+                                           const StdTape<T>* tp = nullptr) {
+    static_assert(!TAPE || (!DENS && !FAST && PASS == 0), "the tape covers the two-pass step");
     constexpr int NT = RX * RY, TX = RX - 4, TY = RY - 4;
     constexpr int SW = RX + 2, SH = RY + 2, NRING = 2 * SW + 2 * (SH - 2);
     // lane t-1 holds region position (pi-1, pj) for every pi >= 1, in the
@@ -265,6 +274,23 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
     // x-neighbour availability for the in-plane max/min (edge copies
     // duplicate the centre and do not change it)
     const bool nxm = gi > 1, nxp = gi < np, nym = gj > 1, nyp = gj < mp;
+    // tape ownership: x/y halo cells of xant and the wall faces n+1 belong to
+    // the tile whose output range touches them
+    bool own_fx = false, own_fy = false, own_s = false;
+    uint32_t tso = 0, tfo = 0;
+    if constexpr (TAPE) {
+        const bool ci = li >= 0 && li < TX && gi >= 1 && gi <= np;
+        const bool cj = lj >= 0 && lj < TY && gj >= 1 && gj <= mp;
+        const bool hi = (gi == 0 && li == -1) || (gi == np + 1 && li >= 0 && li <= TX);
+        const bool hj = (gj == 0 && lj == -1) || (gj == mp + 1 && lj >= 0 && lj <= TY);
+        own_fx = (ci || (gi == np + 1 && li >= 0 && li <= TX)) && cj;
+        own_fy = (cj || (gj == mp + 1 && lj >= 0 && lj <= TY)) && ci;
+        own_s = (ci || hi) && (cj || hj);
+        if (own_s || own_fx || own_fy) {
+            tso = uint32_t(o.s(gi, gj, 1 - o.ih)) * uint32_t(sizeof(T));
+            tfo = uint32_t(o.f1(gi, gj, 1 - o.ih)) * uint32_t(sizeof(T));
+        }
+    }
 
     const int Ks = max(1, k0 - 2), Ke = k1 + 2;
     const int Alast = min(lp, k1 + 2);
@@ -527,6 +553,12 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             }
             sv1[E1][pj][pi] = val1;
             LMID(sv2, E1, 0, 0) = val2;
+            if constexpr (TAPE) {
+                if (P >= k0 && P <= k1) {
+                    if (own_fx) sto(pl(tp->v[0], FXY, P), tfo, val1);
+                    if (own_fy) sto(pl(tp->v[1], YXY, P), tso, val2);
+                }
+            }
         }
         // ---- limited fluxes through the left x-face and lower y-face of this
         //      position on plane Q (V and cp/cn of Q are from the previous
@@ -538,6 +570,8 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             LMID(sf2, 0, 0, 0) = gflux(LMID(sv2, E0, 0, 0), CP(2, li, lj), CN(2, li, lj - 1),
                                 CP(2, li, lj - 1), CN(2, li, lj));
         }
+        if constexpr (TAPE)
+            if (Dq && own_s) sto(pl(tp->psi, SXY, Q), tso, XA(2, li, lj));
         // The loaded source-column queues are not read after this point:
         // the oldest slot of each is loaded with the plane step K+1 needs;
         // the latency is covered by C, D and the next step's barrier.
@@ -646,6 +680,27 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                     if (gj == mp) sto(q2 + o.yx, so, F2p);
                     if (sky) sto(pl(pio->ul3, SXY, Q + 1), so, F3p);
                 }
+                if constexpr (TAPE) {
+                    // V3 on the lower face Q of the cell (face 1 = -face 2 on
+                    // the ground), and on face lp+1 = -face lp on the sky
+                    // plane; cp and cn of the cell; x with the inflow values
+                    // on the wall halo (the upwind's input) and with edge
+                    // copies there (the limiter's bounds)
+                    T* const tv = pl(tp->v[2], SXY, Q);
+                    sto(tv, so, gnd ? -v3q2 : v3q1);
+                    if (sky) sto(tv + SXY, so, -v3q1);
+                    sto(pl(tp->cp, SXY, Q), so, CP(2, li, lj));
+                    sto(pl(tp->cn, SXY, Q), so, CN(2, li, lj));
+                    const T xc = ldo(pl(x, SXY, Q), so);
+                    T* const tb = pl(tp->xb, SXY, Q);
+                    T* const t0 = pl(tp->x0, SXY, Q);
+                    sto(tb, so, xc);
+                    sto(t0, so, xc);
+                    if (gi == 1) { sto(tb - 1, so, bcx[(gj - 1) + mp * (Q - 1)]); sto(t0 - 1, so, xc); }
+                    if (gi == np) { sto(tb + 1, so, bcx[(gj - 1) + mp * ((Q - 1) + lp)]); sto(t0 + 1, so, xc); }
+                    if (gj == 1) { sto(tb - o.sx, so, bcy[(gi - 1) + np * (Q - 1)]); sto(t0 - o.sx, so, xc); }
+                    if (gj == mp) { sto(tb + o.sx, so, bcy[(gi - 1) + np * ((Q - 1) + lp)]); sto(t0 + o.sx, so, xc); }
+                }
             }
             if (gi == 1 || gi == np || gj == 1 || gj == mp) {
                 const T xe = ldo(pl(x, SXY, Q), so);
@@ -753,6 +808,33 @@ k_fused_gauge2(Geo g, Off o, int ntx, int nty, int nchunk, int clen,
         const int k0 = 1 + chunk * clen, k1 = min(g.lp, k0 + clen - 1);
         fused_item<T, RX, RY, false, FAST, PASS>(g, o, ntx, tile, k0, k1, x, u1, u2, u3, rhr, h,
                                                  bcx, bcy, xo, L, nullptr, &pio);
+    }
+}
+
+// The two-pass step that also records the adjoint's tape (fused_item's
+// TAPE), with the work-item order of k_fused_gauge2.
+template <typename T, int RX, int RY, int MINB>
+__global__ void __launch_bounds__(RX * RY, MINB)
+k_fused_gauge_rec(Geo g, Off o, int ntx, int nty, int nchunk, int clen, const T* __restrict__ x,
+                  const T* __restrict__ u1, const T* __restrict__ u2, const T* __restrict__ u3,
+                  const T* __restrict__ rhr, const T* __restrict__ h, const T* __restrict__ bcx,
+                  const T* __restrict__ bcy, T* __restrict__ xo, StdTape<T> tp) {
+    __shared__ FusedLds<T, RX, RY> L;
+    const int ntile = ntx * nty;
+    const int nxcd = gridDim.x >= NXCD ? NXCD : 1;
+    const int per_xcd = gridDim.x / nxcd;
+    const int xcd = blockIdx.x % nxcd, slot = blockIdx.x / nxcd;
+    const int nwg = per_xcd * nxcd;
+    if (blockIdx.x >= nwg) return;
+    const int tiles_per_xcd = (ntile + nxcd - 1) / nxcd;
+    for (int r = slot; ; r += per_xcd) {
+        const int chunk = r / tiles_per_xcd, tix = r - chunk * tiles_per_xcd;
+        if (chunk >= nchunk) break;
+        const int tile = xcd * tiles_per_xcd + tix;
+        if (tile >= ntile) continue;
+        const int k0 = 1 + chunk * clen, k1 = min(g.lp, k0 + clen - 1);
+        fused_item<T, RX, RY, false, false, 0, true>(g, o, ntx, tile, k0, k1, x, u1, u2, u3, rhr,
+                                                     h, bcx, bcy, xo, L, nullptr, nullptr, &tp);
     }
 }
 
@@ -874,6 +956,35 @@ int per_cu_of() {
 // items beat fewer, longer ones; a rounds model over 64..8 planes chose
 // 64 at 256x256x128 and was 7 % slower there), and 8 or 4 planes when the
 // domain is so small that its items still fit one per CU.
+// k-chunk length of least modelled wall time for RX x RY tiles of `kernel`,
+// rounds x (clen + 7): the workgroup slots of the device take the work items
+// in rounds, and an item marches its clen output planes plus the planes
+// recomputed below and above it and its set-up.
+template <int RX, int RY, typename K>
+int chunk_by_rounds(const Geo& g, K kernel) {
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu < 1)
+        ncu = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, RX * RY, 0) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    (void)hipGetLastError();
+    const long long slots = (long long)ncu * per_cu;
+    long long best = -1;
+    int clen = 64;
+    for (int c : {64, 32, 16, 8, 4}) {
+        const long long rounds = (items_of<RX, RY>(g, c) + slots - 1) / slots;
+        const long long cost = rounds * (std::min(c, g.lp) + 7);
+        if (best < 0 || cost < best) {
+            best = cost;
+            clen = c;
+        }
+    }
+    return std::min(clen, g.lp);
+}
+
 template <typename T>
 void choose_tiling(const Geo& g, int& cfg, int& clen) {
     int dev = 0, ncu = 0;

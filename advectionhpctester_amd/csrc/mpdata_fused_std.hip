@@ -638,40 +638,15 @@ hipError_t launch_std(const Geo& g, const Off& o, int clen, const T* x, T* xout,
 
 // Tiling and k-chunk of the standard kernel: 32x16 tiles (the 32x32 build
 // spills registers and measured slower at every size), and the chunk length
-// of least modelled wall time, rounds x (clen + 7): the workgroup slots of
-// the device take the work items in rounds, and an item marches its clen
-// output planes plus the planes recomputed below and above it and its
-// set-up (measured: 128x128x64 78 us at 8 planes against 89 at 16;
-// 64x64x32 4 planes; 256x256x128 and 512x512x256 64 planes).
-// MPDATA_FUSED_CFG / _CHUNK override both for tests and tuning.
+// of least modelled wall time (chunk_by_rounds; measured: 128x128x64 78 us
+// at 8 planes against 89 at 16; 64x64x32 4 planes; 256x256x128 and
+// 512x512x256 64 planes).  MPDATA_FUSED_CFG / _CHUNK override both for
+// tests and tuning.
 template <typename K>
 void choose_std(const Geo& g, K kernel, int& cfg, int& clen) {
     const int env_cfg = fused_config(), env_clen = fused_chunk_env();
     cfg = env_cfg >= 0 ? env_cfg : 1;
-    if (env_clen > 0) {
-        clen = env_clen;
-    } else {
-        int dev = 0, ncu = 0, per_cu = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            ncu < 1)
-            ncu = 256;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 32 * 16, 0) != hipSuccess ||
-            per_cu < 1)
-            per_cu = 1;
-        (void)hipGetLastError();
-        const long long slots = (long long)ncu * per_cu;
-        long long best = -1;
-        clen = 64;
-        for (int c : {64, 32, 16, 8, 4}) {
-            const long long rounds = (items_of<32, 16>(g, c) + slots - 1) / slots;
-            const long long cost = rounds * (std::min(c, g.lp) + 7);
-            if (best < 0 || cost < best) {
-                best = cost;
-                clen = c;
-            }
-        }
-    }
+    clen = env_clen > 0 ? env_clen : chunk_by_rounds<32, 16>(g, kernel);
     if (clen > g.lp) clen = g.lp;
 }
 

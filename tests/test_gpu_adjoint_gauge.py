@@ -8,7 +8,8 @@ limiter bounding the pseudo-velocities, the update divided by h).
   step to rounding).
 * The gauge adjoint kernels give that reference's autograd gradient (x0 and
   the three face velocities) and pass the dot-product test against the
-  tangent-linear gauge step.
+  tangent-linear gauge step; the tape written by the fused gauge step gives
+  the per-stage tape's gradients.
 * The checkpointed autograd Function over the gauge variant equals the
   reference's gradient on the deformational test problem, its taped forward
   equals plain gauge steps over the whole array, and other checkpoint
@@ -120,6 +121,39 @@ def test_gauge_adjoint_matches_reference_gradient(mpd, dims, seed, positive):
         err = _rel(g_[sl], w_[sl])
         print(f"gauge adjoint vs reference gradient {name}: max rel {err:.2e}", flush=True)
         assert err < 1e-12, name
+
+
+@pytest.mark.parametrize("dims,seed,positive,chunk", [((20, 16, 12, 1), 31, True, None),
+                                                      ((13, 18, 7, 1), 32, False, None),
+                                                      ((61, 37, 11, 1), 33, False, "3"),
+                                                      ((61, 37, 11, 1), 34, True, "7"),
+                                                      ((96, 40, 20, 1), 35, False, None)])
+def test_gauge_fused_tape_equals_stage_tape(mpd, dims, seed, positive, chunk, monkeypatch):
+    """The fused gauge recording (the fused gauge step writing the tape, the
+    default path) and the per-stage recording give the same step and the
+    same gradients: every tape value the adjoint kernels read is the same.
+    The adjoint sums scattered contributions with atomics, whose order may
+    differ between runs, so the gradients agree to rounding."""
+    if chunk is not None:
+        monkeypatch.setenv("MPDATA_FUSED_CHUNK", chunk)
+    nsteps = 3
+    c = _case(dims, seed, positive)
+    rng = np.random.default_rng(seed)
+    xbar = _zero_halo(rng.uniform(-1, 1, c["x"].shape))
+    d = {k: to_dev(v) for k, v in c.items()}
+    res = {}
+    for name, path in (("stages", mpd.PATH_STAGES), ("fused", mpd.PATH_AUTO)):
+        with mpd.Domain(*dims) as dom:
+            dom.set_path(path)
+            print(f"gauge tape {name} dims={dims} chunk={chunk}", flush=True)
+            res[name] = _adjoint_chain(mpd, dom, d, nsteps, xbar)
+    (gs, xs), (gf, xf) = res["stages"], res["fused"]
+    assert bits_equal(xs[I], xf[I])
+    for name, a_, b_ in zip(("x", "u1", "u2", "u3"), gs, gf):
+        sl = I if name == "x" else (slice(None),) * 3
+        err = _rel(b_[sl], a_[sl])
+        print(f"fused vs stage gauge tape, gradient {name}: max rel {err:.2e}", flush=True)
+        assert err < 1e-13, name
 
 
 @pytest.mark.parametrize("nsteps", [1, 5])
