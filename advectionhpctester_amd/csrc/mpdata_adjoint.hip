@@ -30,12 +30,38 @@ namespace {
     const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;       \
     const int k = 1 + blockIdx.z;                                  \
     if (i > g.np || j > g.mp || k > g.lp) return;                  \
-    const bool gnd = k == 1, sky = k == g.lp;
+    const bool gnd = k == 1, sky = k == g.lp;                      \
+    const int ci_ = i, cj_ = j, ck_ = k;                           \
+    const Nb nb = nb_at(g, i, j, k);
 
-#define S_(a, ii, jj, kk) a[off_raw<SCAL>(g, ii, jj, kk)]
-#define X_(a, ii, jj, kk) a[off_raw<FACEX>(g, ii, jj, kk)]
-#define Y_(a, ii, jj, kk) a[off_raw<FACEY>(g, ii, jj, kk)]
-#define Z_(a, ii, jj, kk) a[off_raw<FACEZ>(g, ii, jj, kk)]
+// Element offsets around a centre cell (ci_, cj_, ck_) in 32 bits (check_adj
+// keeps every array below 2^31 elements): stencil accesses address their
+// cell or face as the centre's offset plus small constant multiples of the
+// strides, rather than forming a 64-bit offset from the indices each time.
+// z-face arrays share the cell arrays' x/y strides.
+struct Nb {
+    int s, x, y;                    // the centre in cell/z-face, x-face, y-face arrays
+    int sx, sxy, fx, fxy, yx, yxy;  // strides
+};
+__device__ __forceinline__ Nb nb_at(const Geo& g, int i, int j, int k) {
+    Nb n;
+    n.sx = (int)nx_of<SCAL>(g);
+    n.sxy = n.sx * (int)ny_of<SCAL>(g);
+    n.fx = (int)nx_of<FACEX>(g);
+    n.fxy = n.fx * (int)ny_of<FACEX>(g);
+    n.yx = (int)nx_of<FACEY>(g);
+    n.yxy = n.yx * (int)ny_of<FACEY>(g);
+    const int a = i + g.ih - 1, b = j + g.ih - 1, c = k + g.ih - 1;
+    n.s = a + b * n.sx + c * n.sxy;
+    n.x = a + b * n.fx + c * n.fxy;
+    n.y = a + b * n.yx + c * n.yxy;
+    return n;
+}
+#define S_(a, ii, jj, kk) (a)[nb.s + ((ii) - ci_) + ((jj) - cj_) * nb.sx + ((kk) - ck_) * nb.sxy]
+#define X_(a, ii, jj, kk) (a)[nb.x + ((ii) - ci_) + ((jj) - cj_) * nb.fx + ((kk) - ck_) * nb.fxy]
+#define Y_(a, ii, jj, kk) (a)[nb.y + ((ii) - ci_) + ((jj) - cj_) * nb.yx + ((kk) - ck_) * nb.yxy]
+#define Z_(a, ii, jj, kk) (a)[nb.s + ((ii) - ci_) + ((jj) - cj_) * nb.sx + ((kk) - ck_) * nb.sxy]
+// absolute offsets (targets of the LDS flushes, away from any centre)
 #define SP(a, ii, jj, kk) (&a[off_raw<SCAL>(g, ii, jj, kk)])
 #define XP(a, ii, jj, kk) (&a[off_raw<FACEX>(g, ii, jj, kk)])
 #define YP(a, ii, jj, kk) (&a[off_raw<FACEY>(g, ii, jj, kk)])
@@ -49,49 +75,19 @@ __device__ __forceinline__ R sgn(R y) { return y > R(0) ? R(1) : (y < R(0) ? R(-
 __device__ __forceinline__ void acc(R* p, R v) { atomicAdd(p, v); }
 
 // adjoint work arrays held in the context (index into mpdata_ctx::adj)
-enum { A_PSI = 0, A_X0, A_CP, A_CN, A_SUM, A_V1, A_V2, A_V3, A_F1, A_F2, A_F3, A_N };
+enum { A_PSI = 0, A_X0, A_CP, A_CN, A_PSI2, A_V1, A_V2, A_V3, A_F1, A_F2, A_F3, A_N };
 const int A_KIND[A_N] = {SCAL, SCAL, SCAL, SCAL, SCAL, FACEX, FACEY, FACEZ, FACEX, FACEY, FACEZ};
 
+// The arrays that receive scattered contributions (psi, x0, cp, cn, f1..f3)
+// are all zero when a reverse step starts: each is cleared by the kernel
+// that consumes it (psi alternates between two buffers: the upwind's
+// reverse reads psi at its neighbours and clears psi_next, the buffer of the
+// next reverse step).  v1..v3 are written before they are read.
 struct Adj {
-    R *psi, *x0, *cp, *cn, *sum, *v1, *v2, *v3, *f1, *f2, *f3;
-    R uf;  // the velocity adjoint is added to ubar scaled by uf
+    R *psi, *psi_next, *x0, *cp, *cn, *v1, *v2, *v3, *f1, *f2, *f3;
+    const R* xbar;  // adjoint of the recorded step's output
+    R uf;           // the velocity adjoint is added to ubar scaled by uf
 };
-
-// The adjoint arrays at the start of a reverse step, in one launch: the
-// arrays that receive scattered contributions zeroed (blockIdx.y < nz), and
-// the reverse of the update out = rhr*(psi/rhr - sum*hinv)
-// (antidiff3d-standard-gpubc.f90:431-459) in the last row of blocks:
-// psi = w/rhr (zero on the halo) and sum = -(w*hinv) with w = rhr*xbar.
-struct ZeroList {
-    R* p[8];
-    long long n[8];
-    int nz;
-};
-__global__ void k_adj_init(Geo g, ZeroList z, const R* __restrict__ xbar,
-                           const R* __restrict__ rhr, const R* __restrict__ h, Adj a) {
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    if ((int)blockIdx.y < z.nz) {
-        R* __restrict__ p = z.p[blockIdx.y];
-        const long long n = z.n[blockIdx.y];
-        for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += stride)
-            p[q] = R(0);
-        return;
-    }
-    const int ih = g.ih;
-    const long long nx = g.np + 2 * ih, ny = g.mp + 2 * ih, n = len_of<SCAL>(g);
-    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n; q += stride) {
-        const int i = (int)(q % nx) + 1 - ih, j = (int)((q / nx) % ny) + 1 - ih,
-                  k = (int)(q / (nx * ny)) + 1 - ih;
-        if (i >= 1 && i <= g.np && j >= 1 && j <= g.mp && k >= 1 && k <= g.lp) {
-            const R w = rhr[q] * xbar[q];
-            const R hinv = R(1) / h[q];
-            a.psi[q] = w / rhr[q];
-            a.sum[q] = -(w * hinv);
-        } else {
-            a.psi[q] = R(0);
-        }
-    }
-}
 
 // The two x inputs of a recorded step from x_in in one pass: xb with the
 // inflow halo (putbcinxh, mpdataoperators.inc:253-287) for the upwind, x0
@@ -228,7 +224,8 @@ __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict
                                                       const R* __restrict__ cp,
                                                       const R* __restrict__ cn,
                                                       const R* __restrict__ psi,
-                                                      const R* __restrict__ rhr, Adj a) {
+                                                      const R* __restrict__ rhr,
+                                                      const R* __restrict__ h, Adj a) {
     // targets: the cell and its three low neighbours (planes k-1, k)
     constexpr int N = (BX + 1) * (BY + 1) * 2;
     __shared__ R L[3 * N];
@@ -243,49 +240,57 @@ __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     for (int k = kb0; k <= kb1; k++) {
     if (i <= g.np && j <= g.mp) {
+        const int ci_ = i, cj_ = j, ck_ = k;
+        const Nb nb = nb_at(g, i, j, k);
         const bool sky = k == g.lp;
         const R pc = S_(psi, i, j, k), rc = S_(rhr, i, j, k);
-        const R sc = S_(a.sum, i, j, k);
+        // the update's reverse (out = rhr*(psi/rhr - sum*hinv),
+        // antidiff3d-standard-gpubc.f90:431-459): psi's adjoint starts at
+        // w/rhr and sum's is -(w*hinv), w = rhr*xbar
+#define SUMAT(ii, jj, kk) (-((S_(rhr, ii, jj, kk) * S_(a.xbar, ii, jj, kk)) * (R(1) / S_(h, ii, jj, kk))))
+        const R wc = rc * S_(a.xbar, i, j, k);
+        const R sc = -(wc * (R(1) / S_(h, i, j, k)));
+        atomicAdd(&L[rp.at(i, j, k)], wc / rc);
         if (i > 1) {
-            const R fb = S_(a.sum, i - 1, j, k) - sc;
+            const R fb = SUMAT(i - 1, j, k) - sc;
             if constexpr (GAUGE)
                 adj_face_gflux(L, fb, X_(v1, i, j, k), S_(cp, i, j, k), S_(cn, i - 1, j, k),
                                S_(cp, i - 1, j, k), S_(cn, i, j, k), rcp.at(i, j, k), rcn.at(i - 1, j, k),
-                               rcp.at(i - 1, j, k), rcn.at(i, j, k), XP(a.v1, i, j, k));
+                               rcp.at(i - 1, j, k), rcn.at(i, j, k), &X_(a.v1, i, j, k));
             else
             adj_face_flux(L, fb, X_(v1, i, j, k), S_(psi, i - 1, j, k), pc,
                           S_(rhr, i - 1, j, k), rc, S_(cp, i, j, k), S_(cn, i - 1, j, k),
                           S_(cp, i - 1, j, k), S_(cn, i, j, k), rp.at(i - 1, j, k), rp.at(i, j, k),
                           rcp.at(i, j, k), rcn.at(i - 1, j, k), rcp.at(i - 1, j, k),
-                          rcn.at(i, j, k), XP(a.v1, i, j, k));
+                          rcn.at(i, j, k), &X_(a.v1, i, j, k));
         }
         if (j > 1) {
-            const R fb = S_(a.sum, i, j - 1, k) - sc;
+            const R fb = SUMAT(i, j - 1, k) - sc;
             if constexpr (GAUGE)
                 adj_face_gflux(L, fb, Y_(v2, i, j, k), S_(cp, i, j, k), S_(cn, i, j - 1, k),
                                S_(cp, i, j - 1, k), S_(cn, i, j, k), rcp.at(i, j, k), rcn.at(i, j - 1, k),
-                               rcp.at(i, j - 1, k), rcn.at(i, j, k), YP(a.v2, i, j, k));
+                               rcp.at(i, j - 1, k), rcn.at(i, j, k), &Y_(a.v2, i, j, k));
             else
             adj_face_flux(L, fb, Y_(v2, i, j, k), S_(psi, i, j - 1, k), pc,
                           S_(rhr, i, j - 1, k), rc, S_(cp, i, j, k), S_(cn, i, j - 1, k),
                           S_(cp, i, j - 1, k), S_(cn, i, j, k), rp.at(i, j - 1, k), rp.at(i, j, k),
                           rcp.at(i, j, k), rcn.at(i, j - 1, k), rcp.at(i, j - 1, k),
-                          rcn.at(i, j, k), YP(a.v2, i, j, k));
+                          rcn.at(i, j, k), &Y_(a.v2, i, j, k));
         }
         if (k > 1) {
             // the ground cell counts its upper face twice, the sky cell its lower
             const R c1 = k - 1 == 1 ? R(2) : R(1), c2 = sky ? R(2) : R(1);
-            const R fb = c1 * S_(a.sum, i, j, k - 1) - c2 * sc;
+            const R fb = c1 * SUMAT(i, j, k - 1) - c2 * sc;
             if constexpr (GAUGE)
                 adj_face_gflux(L, fb, Z_(v3, i, j, k), S_(cp, i, j, k), S_(cn, i, j, k - 1),
                                S_(cp, i, j, k - 1), S_(cn, i, j, k), rcp.at(i, j, k), rcn.at(i, j, k - 1),
-                               rcp.at(i, j, k - 1), rcn.at(i, j, k), ZP(a.v3, i, j, k));
+                               rcp.at(i, j, k - 1), rcn.at(i, j, k), &Z_(a.v3, i, j, k));
             else
             adj_face_flux(L, fb, Z_(v3, i, j, k), S_(psi, i, j, k - 1), pc,
                           S_(rhr, i, j, k - 1), rc, S_(cp, i, j, k), S_(cn, i, j, k - 1),
                           S_(cp, i, j, k - 1), S_(cn, i, j, k), rp.at(i, j, k - 1), rp.at(i, j, k),
                           rcp.at(i, j, k), rcn.at(i, j, k - 1), rcp.at(i, j, k - 1),
-                          rcn.at(i, j, k), ZP(a.v3, i, j, k));
+                          rcn.at(i, j, k), &Z_(a.v3, i, j, k));
         }
     }
     __syncthreads();
@@ -298,6 +303,7 @@ __global__ void __launch_bounds__(BX * BY) k_adj_flux(Geo g, const R* __restrict
     flush_plane<SCAL>(g, L, rcp, kb1, a.cp, tid);
     flush_plane<SCAL>(g, L, rcn, kb1, a.cn, tid);
 }
+#undef SUMAT
 
 // ---- limiter (:260-347) ----------------------------------------------------
 // max/min over x0 and psi with the forward kernel's order of comparisons;
@@ -343,6 +349,8 @@ __global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restr
     const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
     for (int k = kb0; k <= kb1; k++) {
     if (i <= g.np && j <= g.mp) {
+        const int ci_ = i, cj_ = j, ck_ = k;
+        const Nb nb = nb_at(g, i, j, k);
         const bool gnd = k == 1, sky = k == g.lp;
         const R ep = Eps<R>::v;
         Pick mxo{S_(x0, i - 1, j, k), 0, -1, 0, 0}, mno = mxo;
@@ -434,6 +442,14 @@ __global__ void __launch_bounds__(BX * BY) k_adj_limiter(Geo g, const R* __restr
     flush_plane<SCAL>(g, L, rx, kb1, a.x0, tid);
     flush_plane<SCAL>(g, L, rx, kb1 + 1, a.x0, tid);
     flush_plane<FACEZ>(g, L, r3, kb1 + 1, GAUGE ? a.v3 : a.f3, tid);
+    // cp and cn of the block's cells, read above, are cleared for the next
+    // reverse step (last, so that no load waits for these stores)
+    if (i <= g.np && j <= g.mp)
+        for (int k = kb0; k <= kb1; k++) {
+            const long long q = off_raw<SCAL>(g, i, j, k);
+            a.cp[q] = R(0);
+            a.cn[q] = R(0);
+        }
 }
 
 // ---- pseudo-velocities and first-order fluxes (:72-202) --------------------
@@ -605,6 +621,8 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
     for (int k = kb0; k <= kb1; k++) {
         const bool gnd = k == 1, sky = k == g.lp;
         const bool zterms = !(gnd || sky);
+        const int ci_ = i, cj_ = j, ck_ = k;
+        const Nb nb = nb_at(g, i, j, k);
 #define LPS(ii, jj, kk) (((kk) % 3) * PS + ((jj) - j0 + 1) * (PBX + 2) + ((ii) - i0 + 1))
 #define LU1(ii, jj, kk) (L_U1 + ((kk) % 2) * US + ((jj) - j0 + 1) * (PBX + 1) + ((ii) - i0))
 #define LU2(ii, jj, kk) (L_U2 + ((kk) % 2) * US + ((jj) - j0) * (PBX + 1) + ((ii) - i0 + 1))
@@ -640,10 +658,11 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
             if constexpr (GAUGE)
                 adj_face_gauge(L, X_(a.v1, i, j, k), AXV(i - 1, j, k), AXV(i, j, k), X_(u1, i, j, k), hm,
                                LPS(i - 1, j, k), LPS(i, j, k), LU1(i, j, k), c0, c1, zterms, a.uf);
-            else
-            adj_face_pseudo(L, X_(a.v1, i, j, k), X_(a.f1, i, j, k), X_(v1, i, j, k), AXV(i - 1, j, k),
-                            AXV(i, j, k), X_(u1, i, j, k), hm, LPS(i - 1, j, k), LPS(i, j, k),
-                            LU1(i, j, k), c0, c1, zterms, a.uf);
+            else {
+                adj_face_pseudo(L, X_(a.v1, i, j, k), X_(a.f1, i, j, k), X_(v1, i, j, k),
+                                AXV(i - 1, j, k), AXV(i, j, k), X_(u1, i, j, k), hm,
+                                LPS(i - 1, j, k), LPS(i, j, k), LU1(i, j, k), c0, c1, zterms, a.uf);
+            }
         }
         if (valid && j > 1) {
             const R hm = R(1) / (R(0.5) * (S_(h, i, j - 1, k) + S_(h, i, j, k)));
@@ -658,10 +677,11 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
             if constexpr (GAUGE)
                 adj_face_gauge(L, Y_(a.v2, i, j, k), AXV(i, j - 1, k), AXV(i, j, k), Y_(u2, i, j, k), hm,
                                LPS(i, j - 1, k), LPS(i, j, k), LU2(i, j, k), c0, c1, zterms, a.uf);
-            else
-            adj_face_pseudo(L, Y_(a.v2, i, j, k), Y_(a.f2, i, j, k), Y_(v2, i, j, k), AXV(i, j - 1, k),
-                            AXV(i, j, k), Y_(u2, i, j, k), hm, LPS(i, j - 1, k), LPS(i, j, k),
-                            LU2(i, j, k), c0, c1, zterms, a.uf);
+            else {
+                adj_face_pseudo(L, Y_(a.v2, i, j, k), Y_(a.f2, i, j, k), Y_(v2, i, j, k),
+                                AXV(i, j - 1, k), AXV(i, j, k), Y_(u2, i, j, k), hm,
+                                LPS(i, j - 1, k), LPS(i, j, k), LU2(i, j, k), c0, c1, zterms, a.uf);
+            }
         }
         if (valid && !gnd) {
             const R hm = R(1) / (R(0.5) * (S_(h, i, j, k - 1) + S_(h, i, j, k)));
@@ -675,10 +695,11 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
             if constexpr (GAUGE)
                 adj_face_gauge(L, Z_(a.v3, i, j, k), AXV(i, j, k - 1), AXV(i, j, k), Z_(u3, i, j, k), hm,
                                LPS(i, j, k - 1), LPS(i, j, k), LU3(i, j, k), c0, c1, true, a.uf);
-            else
-            adj_face_pseudo(L, Z_(a.v3, i, j, k), Z_(a.f3, i, j, k), Z_(v3, i, j, k), AXV(i, j, k - 1),
-                            AXV(i, j, k), Z_(u3, i, j, k), hm, LPS(i, j, k - 1), LPS(i, j, k),
-                            LU3(i, j, k), c0, c1, true, a.uf);
+            else {
+                adj_face_pseudo(L, Z_(a.v3, i, j, k), Z_(a.f3, i, j, k), Z_(v3, i, j, k),
+                                AXV(i, j, k - 1), AXV(i, j, k), Z_(u3, i, j, k), hm,
+                                LPS(i, j, k - 1), LPS(i, j, k), LU3(i, j, k), c0, c1, true, a.uf);
+            }
         }
 #undef CORR
 #undef AXV
@@ -697,6 +718,16 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
     flush_psi(kb1 + 1);
     flush_u12(kb1);
     flush_u3(kb1 + 1);
+    // the donor-flux adjoints read above (standard variant) are cleared for
+    // the next reverse step, last so that no load waits for these stores
+    if constexpr (!GAUGE) {
+        if (valid)
+            for (int k = kb0; k <= kb1; k++) {
+                if (i > 1) a.f1[off_raw<FACEX>(g, i, j, k)] = R(0);
+                if (j > 1) a.f2[off_raw<FACEY>(g, i, j, k)] = R(0);
+                if (k > 1) a.f3[off_raw<FACEZ>(g, i, j, k)] = R(0);
+            }
+    }
 }
 
 // ---- upwind (upwind3d-gpubc.f90:101-198) and the sum of both uses of x -----
@@ -711,7 +742,8 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
 // first): its own value plus the x-wall cell it mirrors to, and on the rows
 // that the y walls mirror, the y-wall cell with the x-wall cells that were
 // moved into it.
-__device__ __forceinline__ R psi_in(const Geo& g, const R* __restrict__ p, int i, int j, int k) {
+__device__ __forceinline__ R psi_in(const Geo& g, const Nb& nb, int ci_, int cj_, int ck_,
+                                    const R* __restrict__ p, int i, int j, int k) {
     const int np = g.np, mp = g.mp;
     auto row = [&](int jj) {
         R v = S_(p, i, jj, k);
@@ -727,7 +759,8 @@ __device__ __forceinline__ R psi_in(const Geo& g, const R* __restrict__ p, int i
 
 // The adjoint of the edge-copied x at interior cell (i,j,k): its own value
 // plus the wall halo cells that copy it.
-__device__ __forceinline__ R x0_in(const Geo& g, const R* __restrict__ p, int i, int j, int k) {
+__device__ __forceinline__ R x0_in(const Geo& g, const Nb& nb, int ci_, int cj_, int ck_,
+                                   const R* __restrict__ p, int i, int j, int k) {
     R v = S_(p, i, j, k);
     if (i == 1) v += S_(p, 0, j, k);
     if (i == g.np) v += S_(p, g.np + 1, j, k);
@@ -736,9 +769,11 @@ __device__ __forceinline__ R x0_in(const Geo& g, const R* __restrict__ p, int i,
     return v;
 }
 
-__device__ __forceinline__ R sbar_of(Geo g, const Adj& a, const R* rhr, const R* h, int i, int j,
+__device__ __forceinline__ R sbar_of(const Geo& g, const Nb& nb, int ci_, int cj_, int ck_,
+                                     const Adj& a, const R* rhr, const R* h, int i, int j,
                                      int k) {
-    return -(S_(rhr, i, j, k) * psi_in(g, a.psi, i, j, k) * (R(1) / S_(h, i, j, k)));
+    return -(S_(rhr, i, j, k) * psi_in(g, nb, ci_, cj_, ck_, a.psi, i, j, k) *
+             (R(1) / S_(h, i, j, k)));
 }
 
 __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__ u1,
@@ -751,14 +786,14 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
                                                     R* __restrict__ ub3, Adj a) {
     CELL_HEAD
     const int np = g.np, mp = g.mp, lp = g.lp;
-    const R w = S_(rhr, i, j, k) * psi_in(g, a.psi, i, j, k);
+    const R w = S_(rhr, i, j, k) * psi_in(g, nb, ci_, cj_, ck_, a.psi, i, j, k);
     const R sc = -(w * (R(1) / S_(h, i, j, k)));
-    R xb_c = w + x0_in(g, a.x0, i, j, k);
+    R xb_c = w + x0_in(g, nb, ci_, cj_, ck_, a.x0, i, j, k);
     const R xc = S_(xb, i, j, k);
     // x faces i (c downstream) and i+1 (c upstream)
     {
-        const R fl = (i > 1 ? sbar_of(g, a, rhr, h, i - 1, j, k) : R(0)) - sc;
-        const R fh = sc - (i < np ? sbar_of(g, a, rhr, h, i + 1, j, k) : R(0));
+        const R fl = (i > 1 ? sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i - 1, j, k) : R(0)) - sc;
+        const R fh = sc - (i < np ? sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i + 1, j, k) : R(0));
         const R ul = X_(u1, i, j, k), uh = X_(u1, i + 1, j, k);
         xb_c += fl * (ul < R(0) ? ul : R(0)) + fh * (uh > R(0) ? uh : R(0));
         X_(ub1, i, j, k) += a.uf * (fl * (ind_pos(ul) * S_(xb, i - 1, j, k) + ind_neg(ul) * xc));
@@ -766,8 +801,8 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
             X_(ub1, i + 1, j, k) += a.uf * (fh * (ind_pos(uh) * xc + ind_neg(uh) * S_(xb, i + 1, j, k)));
     }
     {
-        const R fl = (j > 1 ? sbar_of(g, a, rhr, h, i, j - 1, k) : R(0)) - sc;
-        const R fh = sc - (j < mp ? sbar_of(g, a, rhr, h, i, j + 1, k) : R(0));
+        const R fl = (j > 1 ? sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i, j - 1, k) : R(0)) - sc;
+        const R fh = sc - (j < mp ? sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i, j + 1, k) : R(0));
         const R ul = Y_(u2, i, j, k), uh = Y_(u2, i, j + 1, k);
         xb_c += fl * (ul < R(0) ? ul : R(0)) + fh * (uh > R(0) ? uh : R(0));
         Y_(ub2, i, j, k) += a.uf * (fl * (ind_pos(ul) * S_(xb, i, j - 1, k) + ind_neg(ul) * xc));
@@ -778,19 +813,29 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
         // z faces 2..lp; the ground cell counts face 2 twice, the sky cell face lp
         if (!gnd) {
             const R c1 = k - 1 == 1 ? R(2) : R(1), c2 = sky ? R(2) : R(1);
-            const R fl = c1 * sbar_of(g, a, rhr, h, i, j, k - 1) - c2 * sc;
+            const R fl = c1 * sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i, j, k - 1) - c2 * sc;
             const R ul = Z_(u3, i, j, k);
             xb_c += fl * (ul < R(0) ? ul : R(0));
             Z_(ub3, i, j, k) += a.uf * (fl * (ind_pos(ul) * S_(xb, i, j, k - 1) + ind_neg(ul) * xc));
         }
         if (!sky) {
             const R c1 = gnd ? R(2) : R(1), c2 = k + 1 == lp ? R(2) : R(1);
-            const R fh = c1 * sc - c2 * sbar_of(g, a, rhr, h, i, j, k + 1);
+            const R fh = c1 * sc - c2 * sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i, j, k + 1);
             const R uh = Z_(u3, i, j, k + 1);
             xb_c += fh * (uh > R(0) ? uh : R(0));
         }
     }
     S_(xbar, i, j, k) = xb_c;
+    // clear the cells of a.x0 read above (each is read by this cell only) and
+    // this cell's share of the next reverse step's psi buffer: the cell and
+    // the wall halo cells it is the edge of (the only halo cells that
+    // receive contributions); last, so that no load waits for these stores
+    S_(a.x0, i, j, k) = R(0);
+    S_(a.psi_next, i, j, k) = R(0);
+    if (i == 1) { S_(a.x0, 0, j, k) = R(0); S_(a.psi_next, 0, j, k) = R(0); }
+    if (i == np) { S_(a.x0, np + 1, j, k) = R(0); S_(a.psi_next, np + 1, j, k) = R(0); }
+    if (j == 1) { S_(a.x0, i, 0, k) = R(0); S_(a.psi_next, i, 0, k) = R(0); }
+    if (j == mp) { S_(a.x0, i, mp + 1, k) = R(0); S_(a.psi_next, i, mp + 1, k) = R(0); }
 }
 
 #undef CELL_HEAD
@@ -812,6 +857,9 @@ int check_adj(mpdata_ctx* c) {
     if (c->g.px || c->g.py || c->g.pz)
         return fail(MPDATA_EUNSUPPORTED, "the adjoint step covers open boundaries only");
     if (c->g.ih != 1) return fail(MPDATA_EUNSUPPORTED, "the adjoint step needs ih = 1");
+    const long long lim = (1LL << 31) - 1;
+    if (len_of<FACEX>(c->g) >= lim || len_of<FACEY>(c->g) >= lim || len_of<FACEZ>(c->g) >= lim)
+        return fail(MPDATA_EUNSUPPORTED, "the adjoint step needs arrays below 2^31 elements");
     int dev = -1;
     if (hipGetDevice(&dev) == hipSuccess && dev != c->device)
         return fail(MPDATA_EINVAL, "context was created on device %d, current device is %d",
@@ -959,7 +1007,7 @@ void adj_reverse_antidiff(const Geo& g, const TapeSlot& t, const R* u1, const R*
     const R *cp = (const R*)t.cp, *cn = (const R*)t.cn, *psi = (const R*)t.psi;
     const dim3 fgrid((g.np + BX - 1) / BX, (g.mp + BY - 1) / BY, (g.lp + FKB - 1) / FKB);
     const dim3 pgrid((g.np + PBX - 1) / PBX, (g.mp + PBY - 1) / PBY, (g.lp + PKB - 1) / PKB);
-    k_adj_flux<GAUGE><<<fgrid, dim3(BX, BY, 1), 0, s>>>(g, v0, v1, v2, cp, cn, psi, rhr, a);
+    k_adj_flux<GAUGE><<<fgrid, dim3(BX, BY, 1), 0, s>>>(g, v0, v1, v2, cp, cn, psi, rhr, h, a);
     k_adj_limiter<GAUGE><<<fgrid, dim3(BX, BY, 1), 0, s>>>(g, (const R*)t.x0, psi, h, v0, v1, v2,
                                                             cp, cn, a);
     k_adj_pseudo<GAUGE><<<pgrid, dim3(PBX, PBY, 1), 0, s>>>(g, u1, u2, u3, psi, h, v0, v1, v2,
@@ -982,27 +1030,14 @@ int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double*
     const Geo& g = c->g;
     for (int q = 0; q < A_N; q++)
         if ((rc = ensure(c, &c->adj[q], A_KIND[q], 8))) return rc;
-    Adj a{(R*)c->adj[A_PSI], (R*)c->adj[A_X0], (R*)c->adj[A_CP], (R*)c->adj[A_CN],
-          (R*)c->adj[A_SUM], (R*)c->adj[A_V1], (R*)c->adj[A_V2], (R*)c->adj[A_V3],
-          (R*)c->adj[A_F1], (R*)c->adj[A_F2], (R*)c->adj[A_F3], ufac};
+    // psi alternates between A_PSI and A_PSI2 from one reverse step to the
+    // next (the upwind's reverse clears the other buffer)
+    const int pcur = c->adj_psi ? A_PSI2 : A_PSI, pnext = c->adj_psi ? A_PSI : A_PSI2;
+    Adj a{(R*)c->adj[pcur], (R*)c->adj[pnext], (R*)c->adj[A_X0], (R*)c->adj[A_CP],
+          (R*)c->adj[A_CN], (R*)c->adj[A_V1], (R*)c->adj[A_V2], (R*)c->adj[A_V3],
+          (R*)c->adj[A_F1], (R*)c->adj[A_F2], (R*)c->adj[A_F3], xbar, ufac};
     const TapeSlot& t = c->tape[slot];
     hipStream_t s = (hipStream_t)stream;
-    // arrays that receive scattered contributions start at zero, a.psi and
-    // a.sum from the update's reverse; the pseudo-velocity adjoints are
-    // written before they are read
-    {
-        const int zq[6] = {A_X0, A_CP, A_CN, A_F1, A_F2, A_F3};
-        ZeroList z;
-        z.nz = 6;
-        long long nmax = len_of<SCAL>(g);
-        for (int q = 0; q < 6; q++) {
-            z.p[q] = (R*)c->adj[zq[q]];
-            z.n[q] = field_len(g, A_KIND[zq[q]]);
-            nmax = std::max(nmax, z.n[q]);
-        }
-        k_adj_init<<<dim3((unsigned)std::min<long long>(512, (nmax + 255) / 256), 7), 256, 0, s>>>(
-            g, z, xbar, rhr, h, a);
-    }
     HIPCHK(hipGetLastError());
     if (t.variant == MPDATA_GAUGE)
         adj_reverse_antidiff<true>(g, t, u1, u2, u3, rhr, h, ubar1, ubar2, ubar3, a, s);
@@ -1011,6 +1046,7 @@ int mpdata_adj_step_f64(mpdata_ctx* c, int slot, const double* u1, const double*
     k_adj_upwind<<<cgrid(g), cblock(), 0, s>>>(g, u1, u2, u3, (const R*)t.xb, rhr, h, xbar, ubar1,
                                                ubar2, ubar3, a);
     HIPCHK(hipGetLastError());
+    c->adj_psi ^= 1;
     return MPDATA_OK;
 }
 

@@ -47,8 +47,11 @@ struct mpdata_ctx {
     void* red = nullptr;    // reduction partials (device)
     long long red_bytes = 0;
     std::vector<mpdata::TapeSlot> tape;
-    // adjoint work arrays
+    // adjoint work arrays (allocated zeroed; the reverse step leaves them
+    // zeroed) and which of the two xant-adjoint buffers the next reverse
+    // step accumulates into
     void* adj[24] = {};
+    int adj_psi = 0;
     // same-grid multi-field step: output field per batched field, the
     // device descriptor table and the pointers it was packed for
     std::vector<void*> mf_out;
