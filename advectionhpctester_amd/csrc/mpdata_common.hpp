@@ -64,6 +64,10 @@ __host__ __device__ inline long long off(const Geo& g, int i, int j, int k) {
 }
 
 // ---- point operators: advection/src_algorithms/mpdataoperators.inc:1-96 ----
+// XCDs of an MI355X; the workgroups of a launch are dealt to them
+// round-robin (workgroup b to XCD b mod NXCD), each XCD with its own L2.
+constexpr int NXCD = 8;
+
 template <typename T> struct Eps;
 template <> struct Eps<double> { static constexpr double v = 1.e-10; };  // epsilons.f90:4
 template <> struct Eps<float> { static constexpr float v = 1.e-10f; };

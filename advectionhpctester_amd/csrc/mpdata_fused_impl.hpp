@@ -144,8 +144,6 @@ __device__ __forceinline__ T div_(T a, T b) {
 // a chunk of output planes [k0,k1] recomputing xant from k0-2 and cp,cn,V
 // from k0-1.  Inputs of plane K+1 are loaded while plane K is computed.
 // ---------------------------------------------------------------------------
-constexpr int NXCD = 8;  // XCDs of an MI355X; workgroups are dealt to them round-robin
-
 template <typename T>
 struct XSrc {  // where one sx cell comes from: x, bcx, bcy or nothing
     const T* p;
