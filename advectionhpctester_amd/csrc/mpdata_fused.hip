@@ -148,7 +148,7 @@ const char* FusedBatch<T>::config_name(int cfg) {
 
 template <typename T>
 size_t FusedBatch<T>::table_bytes(int n) {
-    return sizeof(Member<T>) * (size_t)n;
+    return sched_offset<T>(n) + sizeof(BatchSched);
 }
 
 template <typename T>

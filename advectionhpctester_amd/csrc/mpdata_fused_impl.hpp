@@ -1145,18 +1145,58 @@ struct Member {  // device descriptor of one member
     T* xo;
 };
 
+// Scheduling state of a batched step, after the member descriptors in the
+// packed table: the run of the work-item list each XCD takes (items
+// bnd[x] .. bnd[x+1]-1, split by modelled cost), and per XCD the counter
+// from which its workgroups take their next item and the count of its
+// workgroups that have finished.  The last workgroup of an XCD to finish
+// resets both, so the next launch (or graph replay) starts from zero with
+// no memset.
+struct BatchSched {
+    int bnd[NXCD + 1];
+    int next[NXCD];
+    int done[NXCD];
+};
+template <typename T>
+size_t sched_offset(int n) {
+    return (sizeof(Member<T>) * (size_t)n + 63) & ~size_t(63);
+}
+
+// Work items are taken dynamically within each XCD's run (items differ in
+// cost: members have different depths, so chunks march different numbers
+// of planes), in list order, so the workgroups of an XCD still march
+// neighbouring tiles together.
 template <typename T, int RX, int RY, int MINB>
 __global__ void __launch_bounds__(RX * RY, MINB)
-k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem, int shared) {
+k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem, int shared,
+                     BatchSched* __restrict__ sc) {
     __shared__ FusedLds<T, RX, RY> L;
+    __shared__ int s_w;
     const int nxcd = gridDim.x >= NXCD ? NXCD : 1;
     const int per_xcd = gridDim.x / nxcd;
-    const int xcd = blockIdx.x % nxcd, slot = blockIdx.x / nxcd;
+    const int xcd = blockIdx.x % nxcd;
     if (blockIdx.x >= per_xcd * nxcd) return;
-    const int items_per_xcd = (nitem + nxcd - 1) / nxcd;
-    for (int r = slot; r < items_per_xcd; r += per_xcd) {
-        const int w = xcd * items_per_xcd + r;
-        if (w >= nitem) break;
+    int w0, w1;  // this XCD's run
+    if (nxcd == 1) {
+        w0 = 0;
+        w1 = nitem;
+    } else if (shared) {
+        // members on one grid: equal shares (the items of one (tile, chunk)
+        // for all members are consecutive and cost the same)
+        const int per = (nitem + NXCD - 1) / NXCD;
+        w0 = min(nitem, xcd * per);
+        w1 = min(nitem, w0 + per);
+    } else {
+        w0 = sc->bnd[xcd];
+        w1 = sc->bnd[xcd + 1];
+    }
+    for (;;) {
+        // every thread reads s_w before the item's barriers, after which
+        // thread 0 may overwrite it
+        if (threadIdx.x == 0) s_w = w0 + atomicAdd(&sc->next[xcd], 1);
+        __syncthreads();
+        const int w = s_w;
+        if (w >= w1) break;
         int q, l;
         if (shared) {
             // members on one grid (fields sharing the velocity): the items
@@ -1179,6 +1219,10 @@ k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem, int
         const int k0 = 1 + chunk * m.clen, k1 = min(m.g.lp, k0 + m.clen - 1);
         fused_item<T, RX, RY>(m.g, m.o, m.ntx, tile, k0, k1, m.x, m.u1, m.u2, m.u3, m.rhr,
                               m.h, m.bcx, m.bcy, m.xo, L);
+    }
+    if (threadIdx.x == 0 && atomicAdd(&sc->done[xcd], 1) == per_xcd - 1) {
+        atomicExch(&sc->next[xcd], 0);
+        atomicExch(&sc->done[xcd], 0);
     }
 }
 
@@ -1237,6 +1281,35 @@ int pack_members(const BatchMember<T>* bm, int n, int clen, Member<T>* out) {
         item += (long long)m.ntile * ((m.g.lp + m.clen - 1) / m.clen);
         if (item > (1ll << 30)) return -1;
     }
+    // XCD runs of equal modelled cost: an item marches its chunk's planes
+    // plus the two recomputed below and above it (fewer at the lids)
+    BatchSched* sc = reinterpret_cast<BatchSched*>(reinterpret_cast<char*>(out) +
+                                                   sched_offset<T>(n));
+    auto cost = [](const Member<T>& m, int chunk) -> long long {
+        const int k0 = 1 + chunk * m.clen, k1 = std::min(m.g.lp, k0 + m.clen - 1);
+        return (long long)(std::min(m.g.lp, k1 + 2) - std::max(1, k0 - 2) + 1);
+    };
+    long long total = 0;
+    for (int q = 0; q < n; q++) {
+        const Member<T>& m = out[q];
+        for (int c = 0; c * m.clen < m.g.lp; c++) total += cost(m, c) * m.ntile;
+    }
+    long long cum = 0;
+    int x = 1;
+    sc->bnd[0] = 0;
+    for (int q = 0; q < n && x < NXCD; q++) {
+        const Member<T>& m = out[q];
+        for (int c = 0; c * m.clen < m.g.lp && x < NXCD; c++) {
+            const long long ci = cost(m, c);
+            for (int t = 0; t < m.ntile && x < NXCD; t++) {
+                cum += ci;
+                while (x < NXCD && cum * NXCD >= total * x)
+                    sc->bnd[x++] = m.item0 + c * m.ntile + t + 1;
+            }
+        }
+    }
+    for (; x <= NXCD; x++) sc->bnd[x] = (int)item;
+    for (int d = 0; d < NXCD; d++) sc->next[d] = sc->done[d] = 0;
     return (int)item;
 }
 
@@ -1250,8 +1323,10 @@ hipError_t launch_batch(const void* table, int n, int nitem, int shared, hipStre
             &per_cu, k_fused_gauge2_batch<T, RX, RY, MINB>, RX * RY, 0);
     if (e != hipSuccess) return e;
     const int grid = persistent_grid(nitem, ((long long)nitem + NXCD - 1) / NXCD, ncu, per_cu);
+    BatchSched* sc = reinterpret_cast<BatchSched*>(
+        reinterpret_cast<char*>(const_cast<void*>(table)) + sched_offset<T>(n));
     k_fused_gauge2_batch<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>((const Member<T>*)table, n, nitem,
-                                                                   shared);
+                                                                   shared, sc);
     return hipGetLastError();
 }
 
