@@ -223,3 +223,37 @@ def test_c4_workload_graph_equals_eager_and_oracle(mpd):
                      (0, 0, 0), shapes[q] + (1,))
         g = _np(graph[q])
         assert bits_equal(g, x), f"member {q} {shapes[q]}: " + describe_diff(g, x)
+
+
+@pytest.mark.parametrize("shapes", [[(8, 8, 4), (12, 9, 5)],
+                                    [(8, 8, 4)] * 3 + [(40, 24, 30), (16, 16, 40)]])
+def test_dynamic_scheduling_small_and_repeated(mpd, shapes):
+    """Work items are taken from per-XCD counters that the last workgroup of
+    each XCD resets: a batch with fewer work items than XCDs (one run for
+    the whole grid), and a mixed batch stepped eagerly five times in a row
+    and as a graph replayed twice, give the single-domain steps each time
+    (a counter left unreset would skip items and leave stale outputs)."""
+    dtype = torch.float64
+    members = _members(dtype, shapes=shapes, seed0=500)
+    ens, xa, xb = _ensemble(mpd, shapes, members, dtype)
+    with ens:
+        for cfg, clen in ((5, 64), (1, 8)):
+            ens.set_tiling(cfg, clen)
+            for q, m in enumerate(members):
+                xa[q].copy_(m["x"])
+            print(f"dynamic scheduling {shapes} cfg {cfg} clen {clen}: eager", flush=True)
+            for n in range(5):
+                ens.step(n & 1)
+            got = [(xb if 5 & 1 else xa)[q].clone() for q in range(len(shapes))]
+            want = _reference_steps(mpd, shapes, members, [m["x"] for m in members], 5, dtype)
+            for q, (g_, w_) in enumerate(zip(got, want)):
+                assert torch.equal(g_, w_), f"eager, cfg {cfg} clen {clen} member {q}"
+            want2 = _reference_steps(mpd, shapes, members, [m["x"] for m in members], 2, dtype)
+            for rep in range(2):
+                print(f"dynamic scheduling {shapes} cfg {cfg}: graph replay {rep}", flush=True)
+                for q, m in enumerate(members):
+                    xa[q].copy_(m["x"])
+                ens.run(2, graph=True)
+                torch.cuda.synchronize()
+                for q, w_ in enumerate(want2):
+                    assert torch.equal(xa[q], w_), f"replay {rep}, cfg {cfg} member {q}"
