@@ -1165,7 +1165,12 @@ size_t sched_offset(int n) {
 // Work items are taken dynamically within each XCD's run (items differ in
 // cost: members have different depths, so chunks march different numbers
 // of planes), in list order, so the workgroups of an XCD still march
-// neighbouring tiles together.
+// neighbouring tiles together.  The items left after the run's whole
+// rounds (one item per workgroup of the XCD) are each split into up to four
+// k-ranges, so that the last round occupies the workgroups; a k-range of a
+// chunk gives the same planes as the whole chunk (8 fields of 256x256x128
+// on one grid: 2.02 -> 1.85 ms per step; in the statically scheduled
+// single-domain kernel the same split measured slower).
 template <typename T, int RX, int RY, int MINB>
 __global__ void __launch_bounds__(RX * RY, MINB)
 k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem, int shared,
@@ -1190,13 +1195,21 @@ k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem, int
         w0 = sc->bnd[xcd];
         w1 = sc->bnd[xcd + 1];
     }
+    const int nrun = w1 - w0, full = nrun / per_xcd * per_xcd, rem = nrun - full;
+    const int split = rem > 0 ? max(1, min(4, per_xcd / rem)) : 1;
+    const int ntake = full + rem * split;
     for (;;) {
-        // every thread reads s_w before the item's barriers, after which
-        // thread 0 may overwrite it
-        if (threadIdx.x == 0) s_w = w0 + atomicAdd(&sc->next[xcd], 1);
+        if (threadIdx.x == 0) s_w = atomicAdd(&sc->next[xcd], 1);
         __syncthreads();
-        const int w = s_w;
-        if (w >= w1) break;
+        const int i = s_w;
+        __syncthreads();  // every thread has read s_w before thread 0 writes it again
+        if (i >= ntake) break;
+        int w = w0 + i, part = 0, nparts = 1;
+        if (i >= full) {
+            w = w0 + full + (i - full) / split;
+            part = (i - full) % split;
+            nparts = split;
+        }
         int q, l;
         if (shared) {
             // members on one grid (fields sharing the velocity): the items
@@ -1216,7 +1229,10 @@ k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem, int
         }
         const Member<T>& m = mem[q];
         const int chunk = l / m.ntile, tile = l - chunk * m.ntile;
-        const int k0 = 1 + chunk * m.clen, k1 = min(m.g.lp, k0 + m.clen - 1);
+        const int c0 = 1 + chunk * m.clen, c1 = min(m.g.lp, c0 + m.clen - 1);
+        const int len = c1 - c0 + 1;
+        const int k0 = c0 + len * part / nparts, k1 = c0 + len * (part + 1) / nparts - 1;
+        if (k0 > k1) continue;  // a chunk shorter than the split (uniform: every thread)
         fused_item<T, RX, RY>(m.g, m.o, m.ntx, tile, k0, k1, m.x, m.u1, m.u2, m.u3, m.rhr,
                               m.h, m.bcx, m.bcy, m.xo, L);
     }
