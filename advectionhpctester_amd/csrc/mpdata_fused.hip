@@ -81,16 +81,17 @@ hipError_t Fused<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const 
     return hipGetLastError();
 }
 
-// 32x16 tiles at 4 waves per SIMD, as the plain step (128 VGPRs with 6
-// spilled for the tape's stores and offsets: 70.5 us per step at
-// 128x128x64 against 79.5 us at 2 waves per SIMD without spills, whose
-// fewer work-item slots force longer chunks), and the chunk of least
-// modelled wall time.
+// 32x16 tiles at 2 waves per SIMD (140 VGPRs, no spill), one workgroup
+// per CU, and the chunk of least modelled wall time.  Alone the kernel is
+// faster at 4 waves per SIMD (6 VGPRs spilled: 70.5 against 79.5 us per
+// step at 128x128x64); but the adjoint's backward runs it beside the
+// reverse kernels on another stream, and there the lighter occupancy
+// leaves them room (forward + backward 16.3-16.5 against 16.7-16.9 ms).
 template <typename T>
 hipError_t Fused<T>::record(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
                             const T* u3, const T* rhr, const T* h, const T* bcx, const T* bcy,
                             const StdTape<T>& tp, hipStream_t s) {
-    constexpr int RX = 32, RY = 16, MINB = 4;
+    constexpr int RX = 32, RY = 16, MINB = 2;
     if (g.px || g.py || g.pz || g.ih != 1) return hipErrorInvalidValue;
     const Off o = make_off(g);
     const int env_clen = fused_chunk_env();
