@@ -51,7 +51,7 @@ struct Fused {
 // contraction and Newton-refined reciprocal division (mpdata_fused_fast.hip).
 // Each step differs from the exactly rounded one by a few units in the last
 // place; after 50 steps of the 512x512x256 workload the normwise relative
-// difference is of order 1e-15.  Same scope, layout and halo results as
+// difference is of order 1e-14.  Same scope, layout and halo results as
 // Fused.
 template <typename T>
 struct FusedFast {

@@ -78,12 +78,12 @@ __device__ __forceinline__ float from_left_lane(float v) {
 __device__ __forceinline__ int clampi(int i, int lo, int hi) { return i < lo ? lo : (i > hi ? hi : i); }
 
 // Reciprocal and quotient of the fast build (FAST = true): the hardware
-// reciprocal (v_rcp) refined by one Newton step e = 1 - b*r, r += e*r, then
-// one correction of the quotient q = a*r, q += (a - b*q)*r: within an ulp
-// or two of the IEEE quotient, in six instructions instead of the ten of
-// the IEEE expansion with its scale and fix-up steps.  (A second Newton
-// step in FP64 left the 50-step difference from the exact build at the
-// same 6.6e-15 and took 1.2 % longer per step at 512x512x256.)  The operands are positive and of order one (h, rhr and the
+// reciprocal (v_rcp) refined by one Newton step e = 1 - b*r, r += e*r, and
+// the quotient a*r: a few ulps from the IEEE quotient, in four
+// instructions instead of the ten of the IEEE expansion with its scale and
+// fix-up steps.  The 50-step difference from the exact build at
+// 512x512x256 is 1.2e-14 (6.6e-15 with a second Newton step and a
+// correction of the quotient, 1.9 % slower), against the 1e-12 tolerance.  The operands are positive and of order one (h, rhr and the
 // limiter denominators, which include eps), far from the ends of the
 // exponent range.  FAST = false is the reference's IEEE division.
 __device__ __forceinline__ double rcp_nr(double b) {
@@ -104,8 +104,7 @@ __device__ __forceinline__ T rcp_(T b) {
 template <bool FAST, typename T>
 __device__ __forceinline__ T div_(T a, T b) {
     if constexpr (FAST) {
-        const T r = rcp_nr(b), q = a * r;
-        return fma_(fma_(-b, q, a), r, q);
+        return a * rcp_nr(b);
     } else {
         return a / b;
     }
@@ -662,8 +661,11 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 sto(pl(dio->rhr, SXY, Q), so, rho_n / xi);
             } else {
                 const T r = rQ;
-                sto(xoq, so, r * (div_<FAST>(XA(2, li, lj), r) -
-                                  div_<FAST>(F1p - F1 + F2p - F2 + F3p - F3, hQ)));
+                if constexpr (FAST)  // r*(xant/r - s/h) = xant - r*s/h, to rounding
+                    sto(xoq, so, XA(2, li, lj) - r * ((F1p - F1 + F2p - F2 + F3p - F3) * rcp_nr(hQ)));
+                else
+                    sto(xoq, so, r * (div_<FAST>(XA(2, li, lj), r) -
+                                      div_<FAST>(F1p - F1 + F2p - F2 + F3p - F3, hQ)));
                 if constexpr (PASS == 1) {
                     // the limited velocities through the low faces of the
                     // cell, and through the wall faces n+1 at the last cells
