@@ -447,7 +447,12 @@ def configs_bench(steps):
         (level n = U*cos(pi t/T), three elementwise products), which
         ms_per_step_with_velocity_update includes; stages_ms_per_step is the
         same step on the per-stage kernels with stored levels and the
-        reference's level rotation.
+        reference's level rotation.  FP64 times the fast build of the fused
+        step (FMA contraction, Newton-refined reciprocals; path "fast"), as
+        the headline does, with the exactly rounded build's time beside it
+        (exact_ms_per_step) and the relative difference of the two after 20
+        steps from the same start, which must stay within 1e-12; FP32 times
+        the exactly rounded build.
     Each entry carries roofline (algorithmic bytes / step time against
     8 TB/s) and cpu_baseline: the oracle (plain C restatement, one thread;
     the reference cannot run these options) on a bounded sample of the same
@@ -536,6 +541,30 @@ def configs_bench(steps):
                     state["n"] += 1
             ms_upd = timed(deform, steps)
             ms = timed(lambda n: deform(n, update=False), steps)
+            fast = None
+            if name == "f64":
+                # the fast build: timed the same way, and its result after 20
+                # steps from the start against the exactly rounded build's
+                def run_from(path, n):
+                    d.set_path(path)
+                    state["n"] = 0
+                    xs[0].copy_(torch.from_numpy(np.ascontiguousarray(x0)).to(dev, tdt))
+                    for lv in lev:
+                        for o, a in zip(lv, U):
+                            o.copy_(a)
+                    deform(n)
+                    torch.cuda.synchronize()
+                    return mpdata.interior(xs[state["n"] & 1]).clone()
+                a_ = run_from(mpdata.PATH_FUSED, 20)
+                b_ = run_from(mpdata.PATH_FAST, 20)
+                err = float((a_ - b_).abs().max() / a_.abs().max())
+                del a_, b_
+                d.set_path(mpdata.PATH_FAST)
+                fast = {"ms_upd": timed(deform, steps),
+                        "ms": timed(lambda n: deform(n, update=False), steps), "err": err}
+                if not err <= 1e-12:
+                    raise SystemExit(f"deformational flow: fast build differs by {err:.3e} "
+                                     "from the exact build after 20 steps (bound 1e-12)")
 
             # the per-stage path with stored levels and the reference's rotation
             d.set_path(mpdata.PATH_STAGES)
@@ -553,9 +582,16 @@ def configs_bench(steps):
                     d.velprd_rotate(uu, vv, ww, ox, oy, oz, h, 1.0)
             ms_st = timed(stages, max(steps // 2, 2))
             del L, ox, oy, oz, uu, vv, ww, xst
-        entry = {"ms_per_step": ms, "value": cells2 / (ms / 1e3), "path": "fused",
-                 "ms_per_step_with_velocity_update": ms_upd, "stages_ms_per_step": ms_st,
-                 "roofline": roof(cells2, ms, bpc)}
+        if fast is not None:
+            entry = {"ms_per_step": fast["ms"], "value": cells2 / (fast["ms"] / 1e3),
+                     "path": "fast", "ms_per_step_with_velocity_update": fast["ms_upd"],
+                     "exact_ms_per_step": ms, "exact_ms_per_step_with_velocity_update": ms_upd,
+                     "max_rel_err_fast_vs_exact_20_steps": fast["err"],
+                     "stages_ms_per_step": ms_st, "roofline": roof(cells2, fast["ms"], bpc)}
+        else:
+            entry = {"ms_per_step": ms, "value": cells2 / (ms / 1e3), "path": "fused",
+                     "ms_per_step_with_velocity_update": ms_upd, "stages_ms_per_step": ms_st,
+                     "roofline": roof(cells2, ms, bpc)}
         if name == "f64":
             npdt = np.float64
             orc = Oracle(npdt)
