@@ -83,6 +83,7 @@ def _load():
     lib.mpdata_ens_config_name.restype = ctypes.c_char_p
     lib.mpdata_ens_set_tiling.argtypes = [_P, _I, _I]
     lib.mpdata_ens_info.argtypes = [_P, IP, IP, IP]
+    lib.mpdata_ens_set_path.argtypes = [_P, _I]
     lib.mpdata_ens_autotune.argtypes = [_P, _I, _P, IP, IP, ctypes.POINTER(_D)]
     lib.mpdata_ens_step.argtypes = [_P, _I, _P]
     lib.mpdata_ens_run.argtypes = [_P, _I, _I, _P]

@@ -22,6 +22,7 @@ struct mpdata_ens {
     std::vector<void*> xa, xb;
     bool have_fields = false;
     int cfg = 0, clen = 64;
+    int fast = 0;  // the fast build of the fused step (MPDATA_PATH_FAST)
     int nitem = 0;
     long long max_halo = 0;
     void* table[2] = {nullptr, nullptr};  // parity 0: xa -> xb, 1: xb -> xa
@@ -105,6 +106,12 @@ int build_tables(mpdata_ens* e, int cfg, int clen) {
 }
 
 hipError_t launch_step(const mpdata_ens* e, int parity, hipStream_t s) {
+    if (e->fast)
+        return e->real_bytes == 8
+                   ? FusedBatchFast<double>::step(e->table[parity], e->n, e->nitem, e->cfg,
+                                                  e->max_halo, s)
+                   : FusedBatchFast<float>::step(e->table[parity], e->n, e->nitem, e->cfg,
+                                                 e->max_halo, s);
     return e->real_bytes == 8
                ? FusedBatch<double>::step(e->table[parity], e->n, e->nitem, e->cfg, e->max_halo, s)
                : FusedBatch<float>::step(e->table[parity], e->n, e->nitem, e->cfg, e->max_halo, s);
@@ -263,6 +270,22 @@ int mpdata_ens_set_tiling(mpdata_ens* e, int cfg, int clen) {
         return MPDATA_OK;
     }
     return build_tables(e, cfg, clen);
+}
+
+int mpdata_ens_set_path(mpdata_ens* e, int path) {
+    int rc = check_ens(e, 0);
+    if (rc) return rc;
+    if (path != MPDATA_PATH_AUTO && path != MPDATA_PATH_FUSED && path != MPDATA_PATH_FAST)
+        return fail(MPDATA_EUNSUPPORTED,
+                    "ensemble: path %d (the batch runs the fused step: AUTO, FUSED or FAST)", path);
+    const int fast = path == MPDATA_PATH_FAST;
+    if (fast != e->fast) {
+        // no replay of a graph of the other build may still run
+        HIPCHK(hipDeviceSynchronize());
+        drop_graph(e);
+        e->fast = fast;
+    }
+    return MPDATA_OK;
 }
 
 int mpdata_ens_info(const mpdata_ens* e, int* nitem, int* cfg, int* clen) {

@@ -122,14 +122,7 @@ template struct FusedDens<double>;
 template struct FusedDens<float>;
 
 // ---- batched step ------------------------------------------------------------
-// Tile configurations of the batched kernel: (id, RX, RY, waves per SIMD).
-#define MPDATA_BATCH_CONFIGS(X) \
-    X(0, 32, 16, 2)             \
-    X(1, 32, 16, 4)             \
-    X(2, 64, 16, 4)             \
-    X(3, 16, 32, 2)             \
-    X(4, 64, 8, 2)              \
-    X(5, 32, 32, 4)
+// (tile configurations: MPDATA_BATCH_CONFIGS in mpdata_fused_impl.hpp)
 
 template <typename T>
 int FusedBatch<T>::nconfigs() {
@@ -181,7 +174,6 @@ hipError_t FusedBatch<T>::step(const void* table, int n, int nitem, int cfg, lon
     k_copy_untouched_halo_batch<T><<<dim3(std::max(bx, 1), n), 256, 0, s>>>((const Member<T>*)table);
     return hipGetLastError();
 }
-#undef MPDATA_BATCH_CONFIGS
 
 // Batch tiling for n members on one grid (configuration numbers of
 // MPDATA_BATCH_CONFIGS): the single-domain rule with n times the work items.

@@ -133,6 +133,17 @@ struct BatchMember {
 // launch (plus one halo-copy launch).  The caller packs a descriptor table
 // on the host (pack), copies it to device memory and launches with it.
 template <typename T>
+struct FusedBatch;
+
+// The batched step of the fast build (mpdata_fused_fast.hip): the same
+// tables, tilings and halo copy as FusedBatch, the fused item of FusedFast.
+template <typename T>
+struct FusedBatchFast {
+    static hipError_t step(const void* table, int n, int nitem, int cfg, long long max_halo,
+                           hipStream_t s, int shared = 0);
+};
+
+template <typename T>
 struct FusedBatch {
     // number of tile configurations and a description of each
     static int nconfigs();

@@ -32,7 +32,25 @@ hipError_t FusedFast<T>::step(const Geo& g, const T* x, T* xout, const T* u1, co
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t FusedBatchFast<T>::step(const void* table, int n, int nitem, int cfg,
+                                   long long max_halo, hipStream_t s, int shared) {
+    hipError_t e = hipErrorInvalidValue;
+    switch (cfg) {
+#define LAUNCH(c, rx, ry, mb) \
+    case c: e = launch_batch<T, rx, ry, mb, true>(table, n, nitem, shared, s); break;
+        MPDATA_BATCH_CONFIGS(LAUNCH)
+#undef LAUNCH
+    }
+    if (e != hipSuccess) return e;
+    const int bx = (int)std::min<long long>(64, (max_halo + 255) / 256);
+    k_copy_untouched_halo_batch<T><<<dim3(std::max(bx, 1), n), 256, 0, s>>>((const Member<T>*)table);
+    return hipGetLastError();
+}
+
 template struct FusedFast<double>;
 template struct FusedFast<float>;
+template struct FusedBatchFast<double>;
+template struct FusedBatchFast<float>;
 
 }  // namespace mpdata

@@ -1171,7 +1171,7 @@ size_t sched_offset(int n) {
 // chunk gives the same planes as the whole chunk (8 fields of 256x256x128
 // on one grid: 2.02 -> 1.85 ms per step; in the statically scheduled
 // single-domain kernel the same split measured slower).
-template <typename T, int RX, int RY, int MINB>
+template <typename T, int RX, int RY, int MINB, bool FAST = false>
 __global__ void __launch_bounds__(RX * RY, MINB)
 k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem, int shared,
                      BatchSched* __restrict__ sc) {
@@ -1233,8 +1233,8 @@ k_fused_gauge2_batch(const Member<T>* __restrict__ mem, int nmem, int nitem, int
         const int len = c1 - c0 + 1;
         const int k0 = c0 + len * part / nparts, k1 = c0 + len * (part + 1) / nparts - 1;
         if (k0 > k1) continue;  // a chunk shorter than the split (uniform: every thread)
-        fused_item<T, RX, RY>(m.g, m.o, m.ntx, tile, k0, k1, m.x, m.u1, m.u2, m.u3, m.rhr,
-                              m.h, m.bcx, m.bcy, m.xo, L);
+        fused_item<T, RX, RY, false, FAST>(m.g, m.o, m.ntx, tile, k0, k1, m.x, m.u1, m.u2, m.u3,
+                                           m.rhr, m.h, m.bcx, m.bcy, m.xo, L);
     }
     if (threadIdx.x == 0 && atomicAdd(&sc->done[xcd], 1) == per_xcd - 1) {
         atomicExch(&sc->next[xcd], 0);
@@ -1329,20 +1329,30 @@ int pack_members(const BatchMember<T>* bm, int n, int clen, Member<T>* out) {
     return (int)item;
 }
 
-template <typename T, int RX, int RY, int MINB>
+// Tile configurations of the batched kernel: (id, RX, RY, waves per SIMD),
+// shared by the exact and the fast build.
+#define MPDATA_BATCH_CONFIGS(X) \
+    X(0, 32, 16, 2)             \
+    X(1, 32, 16, 4)             \
+    X(2, 64, 16, 4)             \
+    X(3, 16, 32, 2)             \
+    X(4, 64, 8, 2)              \
+    X(5, 32, 32, 4)
+
+template <typename T, int RX, int RY, int MINB, bool FAST = false>
 hipError_t launch_batch(const void* table, int n, int nitem, int shared, hipStream_t s) {
     int dev = 0, ncu = 0, per_cu = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, k_fused_gauge2_batch<T, RX, RY, MINB>, RX * RY, 0);
+            &per_cu, k_fused_gauge2_batch<T, RX, RY, MINB, FAST>, RX * RY, 0);
     if (e != hipSuccess) return e;
     const int grid = persistent_grid(nitem, ((long long)nitem + NXCD - 1) / NXCD, ncu, per_cu);
     BatchSched* sc = reinterpret_cast<BatchSched*>(
         reinterpret_cast<char*>(const_cast<void*>(table)) + sched_offset<T>(n));
-    k_fused_gauge2_batch<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>((const Member<T>*)table, n, nitem,
-                                                                   shared, sc);
+    k_fused_gauge2_batch<T, RX, RY, MINB, FAST><<<grid, RX * RY, 0, s>>>(
+        (const Member<T>*)table, n, nitem, shared, sc);
     return hipGetLastError();
 }
 

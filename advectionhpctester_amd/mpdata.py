@@ -499,6 +499,12 @@ class Ensemble:
     def set_tiling(self, cfg, clen):
         check(LIB.mpdata_ens_set_tiling(self._e, cfg, clen))
 
+    def set_path(self, path):
+        """PATH_AUTO / PATH_FUSED (the exactly rounded fused step, the
+        default) or PATH_FAST (the fast build, equal to Domain steps on
+        PATH_FAST and within 1e-12 of the exact build)."""
+        check(LIB.mpdata_ens_set_path(self._e, int(path)))
+
     def info(self):
         """(work items per step, tile configuration, k-chunk length)."""
         a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -304,7 +304,10 @@ def ensemble_bench(steps, nmember=64, seed=4):
     direction, Gaussian blobs, per-tracer solid-body rotation (max Courant
     0.3-0.8), FP64 gauge MPDATA, 2 iterations, limiter on: `steps` steps as
     one HIP graph of batched persistent launches (tiling autotuned first)
-    against the same steps as 64 single-domain launches per step."""
+    against the same steps as 64 single-domain launches per step.  The fast
+    build of the fused step is timed first (as the headline), then the
+    exactly rounded one (exact_*); the two differ by at most 1e-12 (relative)
+    after 10 steps from the same start, checked here."""
     import torch
 
     from advectionhpctester_amd import mpdata, testdata
@@ -314,20 +317,37 @@ def ensemble_bench(steps, nmember=64, seed=4):
                for q, sh in enumerate(shapes)]
     cells = sum(p * q * r for p, q, r in shapes)
     xa = [m["x"] for m in members]
+    x0 = [t.clone() for t in xa]
     xb = [torch.empty_like(t) for t in xa]
     rest = [[m[k] for m in members] for k in ("u1", "u2", "u3", "rhr", "h", "bcx", "bcy")]
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {}
     with mpdata.Ensemble(shapes) as ens:
         ens.set_fields(xa, xb, *rest)
-        cfg, clen, _ = ens.autotune(trial_steps=2)
-        ens.run(steps, graph=True)           # records the graph
-        torch.cuda.synchronize()
-        ev0.record()
-        ens.run(steps, graph=True)
-        ev1.record()
-        torch.cuda.synchronize()
-        ms_graph = ev0.elapsed_time(ev1)
-        nitem = ens.info()[0]
+        for name, path in (("fast", mpdata.PATH_FAST), ("exact", mpdata.PATH_FUSED)):
+            ens.set_path(path)
+            cfg, clen, _ = ens.autotune(trial_steps=2)
+            ens.run(steps, graph=True)           # records the graph
+            torch.cuda.synchronize()
+            ev0.record()
+            ens.run(steps, graph=True)
+            ev1.record()
+            torch.cuda.synchronize()
+            out[name] = (ev0.elapsed_time(ev1), cfg, clen, ens.info()[0])
+        ends = {}
+        for name, path in (("fast", mpdata.PATH_FAST), ("exact", mpdata.PATH_FUSED)):
+            ens.set_path(path)
+            for a, b in zip(xa, x0):
+                a.copy_(b)
+            ens.run(10, graph=False)
+            torch.cuda.synchronize()
+            ends[name] = [mpdata.interior(t).clone() for t in xa]
+        err = max(float((f - e).abs().max() / e.abs().max())
+                  for f, e in zip(ends["fast"], ends["exact"]))
+        del ends
+    if not err <= 1e-12:
+        raise SystemExit(f"ensemble: fast build differs by {err:.3e} from the exact build "
+                         "after 10 steps (bound 1e-12)")
     doms = [mpdata.Domain(*sh) for sh in shapes]
     fields = [(m["u1"], m["u2"], m["u3"], m["rhr"], m["h"], m["bcx"], m["bcy"]) for m in members]
 
@@ -339,24 +359,38 @@ def ensemble_bench(steps, nmember=64, seed=4):
                 else:
                     d.step_oop(a, b, *f)
 
-    sequential(2)
-    torch.cuda.synchronize()
-    ev0.record()
-    sequential(steps)
-    ev1.record()
-    torch.cuda.synchronize()
-    ms_seq = ev0.elapsed_time(ev1)
+    seq = {}
+    for name, path in (("fast", mpdata.PATH_FAST), ("exact", mpdata.PATH_AUTO)):
+        for d in doms:
+            d.set_path(path)
+        sequential(2)
+        torch.cuda.synchronize()
+        ev0.record()
+        sequential(steps)
+        ev1.record()
+        torch.cuda.synchronize()
+        seq[name] = ev0.elapsed_time(ev1)
     for d in doms:
         d.close()
     alg = 56 * cells
-    return {"members": nmember, "cells": cells, "steps": steps,
+    ms_graph, cfg, clen, nitem = out["fast"]
+    ms_ex, cfg_ex, clen_ex, _ = out["exact"]
+    return {"members": nmember, "cells": cells, "steps": steps, "path": "fast",
             "tiling": mpdata.Ensemble.configs()[cfg], "k_chunk": clen, "work_items": nitem,
             "graph_ms_per_step": ms_graph / steps,
             "graph_value": cells * steps / (ms_graph / 1e3),
             "graph_hbm_alg_GBs": alg / (ms_graph / steps / 1e3) / 1e9,
-            "sequential_ms_per_step": ms_seq / steps,
-            "sequential_value": cells * steps / (ms_seq / 1e3),
-            "speedup_graph_vs_sequential": ms_seq / ms_graph}
+            "graph_frac": alg / (ms_graph / steps / 1e3) / 8e12,
+            "sequential_ms_per_step": seq["fast"] / steps,
+            "sequential_value": cells * steps / (seq["fast"] / 1e3),
+            "speedup_graph_vs_sequential": seq["fast"] / ms_graph,
+            "max_rel_err_fast_vs_exact_10_steps": err,
+            "exact": {"tiling": mpdata.Ensemble.configs()[cfg_ex], "k_chunk": clen_ex,
+                      "graph_ms_per_step": ms_ex / steps,
+                      "graph_value": cells * steps / (ms_ex / 1e3),
+                      "graph_frac": alg / (ms_ex / steps / 1e3) / 8e12,
+                      "sequential_ms_per_step": seq["exact"] / steps,
+                      "speedup_graph_vs_sequential": seq["exact"] / ms_ex}}
 
 
 def multifield_bench(steps, size=(256, 256, 128), nfields=8):

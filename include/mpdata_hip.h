@@ -497,6 +497,13 @@ int mpdata_ens_nconfigs(void);
 const char *mpdata_ens_config_name(int cfg);
 int mpdata_ens_set_tiling(mpdata_ens *ens, int cfg, int clen);
 int mpdata_ens_info(const mpdata_ens *ens, int *nitem, int *cfg, int *clen);
+/* Which build of the fused step the batch runs: MPDATA_PATH_AUTO or
+ * MPDATA_PATH_FUSED (the exactly rounded one, the default; results
+ * bit-identical to mpdata_step_oop_*), or MPDATA_PATH_FAST (FMA contraction
+ * and Newton-refined reciprocals; results bit-identical to mpdata_step_oop_*
+ * on a context set to MPDATA_PATH_FAST, and within 1e-12 of the exact
+ * build).  A recorded graph is dropped when the build changes. */
+int mpdata_ens_set_path(mpdata_ens *ens, int path);
 /* Times every (configuration, k-chunk length) on trial_steps steps of the
  * real fields and keeps the fastest; the choice is cached per set of
  * shapes for the process.  Reads xa and overwrites xb.  Synchronous. */

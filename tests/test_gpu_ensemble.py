@@ -257,3 +257,40 @@ def test_dynamic_scheduling_small_and_repeated(mpd, shapes):
                 torch.cuda.synchronize()
                 for q, w_ in enumerate(want2):
                     assert torch.equal(xa[q], w_), f"replay {rep}, cfg {cfg} member {q}"
+
+
+def test_fast_build_equals_fast_single_domain_steps(mpd):
+    """The batch on the fast build (mpdata_ens_set_path(MPDATA_PATH_FAST))
+    gives the bits of single-domain steps on the fast build, every tiling,
+    over three steps, and stays within 1e-12 of the exact batch."""
+    dtype = torch.float64
+    members = _members(dtype, seed0=700)
+    nsteps = 3
+    want = []
+    for s, m in zip(SHAPES, members):
+        with mpd.Domain(*s, dtype=dtype) as d:
+            d.set_path(mpd.PATH_FAST)
+            a, b = m["x"].clone(), torch.empty_like(m["x"])
+            for _ in range(nsteps):
+                d.step_oop(a, b, m["u1"], m["u2"], m["u3"], m["rhr"], m["h"], m["bcx"], m["bcy"])
+                a, b = b, a
+            want.append(a)
+    exact = _reference_steps(mpd, SHAPES, members, [m["x"] for m in members], nsteps, dtype)
+    ens, xa, xb = _ensemble(mpd, SHAPES, members, dtype)
+    with ens:
+        ens.set_path(mpd.PATH_FAST)
+        for cfg in range(len(mpd.Ensemble.configs())):
+            print(f"fast ensemble cfg {cfg}", flush=True)
+            ens.set_tiling(cfg, 16)
+            for q, m in enumerate(members):
+                xa[q].copy_(m["x"])
+            ens.run(nsteps, graph=False)
+            torch.cuda.synchronize()
+            got = xb if nsteps & 1 else xa
+            for q, (g_, w_, e_) in enumerate(zip(got, want, exact)):
+                assert torch.equal(g_, w_), f"cfg {cfg} member {q} {SHAPES[q]}"
+                i = (slice(1, -1),) * 3
+                err = float((g_[i] - e_[i]).abs().max() / e_[i].abs().max())
+                assert err <= 1e-12, f"cfg {cfg} member {q}: {err:.2e} from the exact build"
+        with pytest.raises(mpd.MpdataError):
+            ens.set_path(mpd.PATH_STAGES)
