@@ -13,9 +13,10 @@
 // Here every gauge field runs in one launch of the batched fused kernel in
 // its shared-grid mode: the work items of one tile and k-chunk are
 // consecutive for all fields, so one XCD marches them together and reads
-// u1, u2, u3, h and rhr into its L2 once for all of them.  Standard fields,
-// and every field when the fused kernel does not apply, take per-field
-// steps.
+// u1, u2, u3, h and rhr into its L2 once for all of them (on the fast build
+// when the context is set to MPDATA_PATH_FAST, as mpdata_step_* is).
+// Standard fields, and every field when the fused kernel does not apply,
+// take per-field steps.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -182,7 +183,11 @@ int ce_impl(mpdata_ctx* c, int alg_choice, int liner, int nfields, const int* ro
     if (rc) return rc;
     const long long nx = c->g.np + 2, ny = c->g.mp + 2;
     const long long halo = 2 * nx * ny + 2 * nx * c->g.lp + 2ll * c->g.mp * c->g.lp;
-    HIPCHK(FusedBatch<T>::step(c->mf_table, n, c->mf_nitem, c->mf_cfg, halo, s, 1));
+    // the build of the fused step mpdata_step_* runs on this context
+    if (c->path == MPDATA_PATH_FAST)
+        HIPCHK(FusedBatchFast<T>::step(c->mf_table, n, c->mf_nitem, c->mf_cfg, halo, s, 1));
+    else
+        HIPCHK(FusedBatch<T>::step(c->mf_table, n, c->mf_nitem, c->mf_cfg, halo, s, 1));
     if (!oop)
         for (int j = 0; j < n; j++)
             HIPCHK(hipMemcpyAsync(x_out[batched[j]], c->mf_out[j], bytes, hipMemcpyDeviceToDevice, s));

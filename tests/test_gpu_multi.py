@@ -78,8 +78,10 @@ def test_fields_equal_single_steps_and_oracle(mp, tdt, npdt, alg_choice):
                 f"field {f}: " + describe_diff(interior(got), interior(want[f]))
 
 
-@pytest.mark.parametrize("path_name", ["PATH_AUTO", "PATH_STAGES"])
+@pytest.mark.parametrize("path_name", ["PATH_AUTO", "PATH_STAGES", "PATH_FAST"])
 def test_out_of_place_and_repeated_calls(mp, path_name):
+    """Against one field at a time on the same path (on PATH_FAST the gauge
+    fields' batch runs the fast build, as mpdata_step does there)."""
     dims = (40, 24, 18, 1)
     nf = 6
     base, xs, bxs, bys = _fields(dims, nf, np.float64, seed=3)
