@@ -35,6 +35,7 @@ adjoint (forward+backward against one forward).  --density-only,
 alone.
 """
 import argparse
+import faulthandler
 import json
 import os
 import sys
@@ -108,6 +109,8 @@ def copy_probe(a):
 
 
 def main():
+    # a crash of the process (host segfault) prints the Python stack on stderr
+    faulthandler.enable()
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
