@@ -30,7 +30,8 @@ Sub-objects: unfused_stages (the per-stage kernels), ensemble (64 uneven
 tracers in one persistent launch / HIP graph), multifield (eight fields on one
 grid), density (the density step with the velocity predictor), configs (the
 periodic-standard and deformational-flow configurations, fused kernels),
-adjoint (forward+backward against one forward).  --density-only,
+adjoint (forward+backward against one forward); the ensemble, configs and
+adjoint objects carry their own cpu_baseline.  --density-only,
 --configs-only, --multifield-only and --adjoint-only print that object
 alone.
 """
@@ -237,7 +238,7 @@ def main():
         del bufs, args, case
         dom.close()
         torch.cuda.empty_cache()
-        ens = ensemble_bench(a.ensemble_steps)
+        ens = ensemble_bench(a.ensemble_steps, cpu=a.cpu_steps is None or a.cpu_steps > 0)
 
     multi = None
     if not a.no_multifield and a.dtype == "f64":
@@ -257,7 +258,7 @@ def main():
     adj = None
     if not a.no_adjoint and a.dtype == "f64":
         torch.cuda.empty_cache()
-        adj = adjoint_bench()
+        adj = adjoint_bench(cpu=a.cpu_steps is None or a.cpu_steps > 0)
 
     out = {
         "metric": METRIC,
@@ -302,7 +303,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def ensemble_bench(steps, nmember=64, seed=4):
+def ensemble_bench(steps, nmember=64, seed=4, cpu=True):
     """64 tracers with per-tracer grids drawn from 32..256 (step 16) in each
     direction, Gaussian blobs, per-tracer solid-body rotation (max Courant
     0.3-0.8), FP64 gauge MPDATA, 2 iterations, limiter on: `steps` steps as
@@ -310,7 +311,8 @@ def ensemble_bench(steps, nmember=64, seed=4):
     against the same steps as 64 single-domain launches per step.  The fast
     build of the fused step is timed first (as the headline), then the
     exactly rounded one (exact_*); the two differ by at most 1e-12 (relative)
-    after 10 steps from the same start, checked here."""
+    after 10 steps from the same start, checked here.  cpu: one step of the
+    first members on the CPU (ensemble_cpu_baseline)."""
     import torch
 
     from advectionhpctester_amd import mpdata, testdata
@@ -375,6 +377,7 @@ def ensemble_bench(steps, nmember=64, seed=4):
         seq[name] = ev0.elapsed_time(ev1)
     for d in doms:
         d.close()
+    cpu_entry = ensemble_cpu_baseline(x0, rest, shapes) if cpu else None
     alg = 56 * cells
     ms_graph, cfg, clen, nitem = out["fast"]
     ms_ex, cfg_ex, clen_ex, _ = out["exact"]
@@ -388,12 +391,42 @@ def ensemble_bench(steps, nmember=64, seed=4):
             "sequential_value": cells * steps / (seq["fast"] / 1e3),
             "speedup_graph_vs_sequential": seq["fast"] / ms_graph,
             "max_rel_err_fast_vs_exact_10_steps": err,
+            "cpu_baseline": cpu_entry,
             "exact": {"tiling": mpdata.Ensemble.configs()[cfg_ex], "k_chunk": clen_ex,
                       "graph_ms_per_step": ms_ex / steps,
                       "graph_value": cells * steps / (ms_ex / 1e3),
                       "graph_frac": alg / (ms_ex / steps / 1e3) / 8e12,
                       "sequential_ms_per_step": seq["exact"] / steps,
                       "speedup_graph_vs_sequential": seq["exact"] / ms_ex}}
+
+
+def ensemble_cpu_baseline(x0, rest, shapes, max_cells=25_000_000):
+    """One step of the first members (up to max_cells cells) from their
+    initial fields with the C oracle (oracle/liboracle_mpdata.so, one
+    thread), a member at a time.  Not the reference build: its y-wall
+    zeroing (defines.inc:208-216) writes row np+1 of the y-face array, past
+    the array when np > mp+1, which most members are, so it is run on square
+    planes only."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+
+    orc = Oracle(np.float64)
+    cells, dt, n = 0, 0.0, 0
+    for q, (np_, mp_, lp_) in enumerate(shapes):
+        if cells >= max_cells:
+            break
+        x = np.ascontiguousarray(x0[q].cpu().numpy())
+        u1, u2, u3, rhr, h, bcx, bcy = (np.ascontiguousarray(r[q].cpu().numpy()) for r in rest)
+        t0 = time.perf_counter()
+        orc.step(1, 2, 1, x, u1, u2, u3, rhr, h, bcx, bcy, (0, 0, 0), (np_, mp_, lp_, 1))
+        dt += time.perf_counter() - t0
+        cells += np_ * mp_ * lp_
+        n += 1
+    return {"value": cells / dt, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"one step of each of the first {n} of the {len(shapes)} members ({cells} "
+                      f"cells) with oracle/liboracle_mpdata.so (plain C restatement, -O2, one "
+                      f"thread), {dt:.1f} s"}
 
 
 def multifield_bench(steps, size=(256, 256, 128), nfields=8):
@@ -715,18 +748,56 @@ def density_bench(steps, size=(256, 256, 128)):
     return out
 
 
-def adjoint_bench(nsteps=50, ckpt=10, reps=3):
+def adjoint_bench(nsteps=50, ckpt=10, reps=3, cpu=True):
     """Gradient of J = 0.5 |x_T - obs|^2 after `nsteps` standard
     non-oscillatory steps at 128x128x64 under the reversing deformational
     flow, with respect to x0 and the face velocities, checkpoints every
     `ckpt` steps: time of forward + backward against one plain forward of
     the same steps (both include forming each step's velocity).  The same
-    measurement over gauge steps goes under "gauge"."""
+    measurement over gauge steps goes under "gauge".  cpu: the same gradient
+    by reverse-mode differentiation on the CPU (adjoint_cpu_baseline)."""
     out = _adjoint_cost("standard", nsteps, ckpt, reps)
     out["target_ratio"] = 4.0
     g = _adjoint_cost("gauge", nsteps, ckpt, reps)
     out["gauge"] = {k: g[k] for k in ("forward_ms", "forward_backward_ms", "ratio")}
+    out["cpu_baseline"] = adjoint_cpu_baseline(nsteps) if cpu else None
     return out
+
+
+def adjoint_cpu_baseline(nsteps, sample=2):
+    """The reference has no adjoint; the CPU path for the gradient is
+    torch.autograd over oracle/mpdata_torch.py (the oracle's standard step
+    in vectorised torch operations, every intermediate kept, no checkpoints)
+    on one thread: forward and backward of the first `sample` steps of the
+    same problem, scaled to `nsteps` steps."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mpdata_torch
+
+    from advectionhpctester_amd import testdata
+    c = testdata.adjoint_case(128, 128, 64, nsteps=nsteps)
+    f = c.pop("factors")
+    obs = torch.from_numpy(c.pop("obs"))
+    t = {k: torch.from_numpy(v) for k, v in c.items()}
+    threads = torch.get_num_threads()
+    torch.set_num_threads(1)
+    try:
+        leaves = [t[k].clone().requires_grad_(True) for k in ("x", "u1", "u2", "u3")]
+        t0 = time.perf_counter()
+        y = leaves[0]
+        for n in range(sample):
+            y = mpdata_torch.step(y, leaves[1] * f[n], leaves[2] * f[n], leaves[3] * f[n],
+                                  t["rhr"], t["h"], t["bcx"], t["bcy"])
+        r = (y - obs)[1:-1, 1:-1, 1:-1]
+        (0.5 * (r * r).sum()).backward()
+        dt = time.perf_counter() - t0
+    finally:
+        torch.set_num_threads(threads)
+    return {"value": dt * 1e3 * nsteps / sample, "unit": f"ms per forward+backward of {nsteps} steps",
+            "cores": 1, "kind": "port",
+            "sample": f"{sample} steps forward + backward with torch.autograd on the CPU over "
+                      f"oracle/mpdata_torch.py (the oracle's standard step in torch operations, "
+                      f"one thread), {dt:.1f} s, scaled to {nsteps} steps"}
 
 
 def _adjoint_cost(variant_name, nsteps, ckpt, reps):
