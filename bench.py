@@ -114,7 +114,7 @@ def main():
     faulthandler.enable()
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+        print(f"bench.py: the library runs on one GPU; --gpus {a.gpus} is not supported "
               "(the step would shard by 3-D domain decomposition, one block per GPU)", file=sys.stderr)
         sys.exit(2)
     if a.copy_probe:

@@ -6,7 +6,7 @@ independent reference gradient for the hand-written adjoint kernels.
 The operations and their order follow the oracle (mpdata_oracle_impl.inc),
 which is pinned to the reference: upwind3d-gpubc.f90:101-198,
 antidiff3d-standard-gpubc.f90:59-462 and antidiff3d-gauge-gpubc.f90:71-430
-(with this build's ground-plane bound, the 6-point minimum of
+(with the oracle's ground-plane bound, the 6-point minimum of
 mpdata3d-rho-gauge-gpubc.f90:365-368).  Every branch is a torch.where on the
 forward value (|y| has derivative 0 at y = 0), the linearisation the
 tangent-linear and adjoint kernels use.  Arrays are (k, j, i) with Fortran
