@@ -731,6 +731,8 @@ __global__ void __launch_bounds__(PBX * PBY) k_adj_pseudo(Geo g, const R* __rest
 }
 
 // ---- upwind (upwind3d-gpubc.f90:101-198) and the sum of both uses of x -----
+// One thread per cell, UBX x UBY cells of one plane per block.
+constexpr int UBX = 64, UBY = 4;
 // The reverse of the halo fills (mirror of xant, edge copy of x) is folded
 // in: psi_in and x0_in gather the halo cells' adjoints.
 // xant = rhr*(x - sum*hinv), fluxes donor(x(m), x(c), u); the x halo holds the
@@ -776,7 +778,7 @@ __device__ __forceinline__ R sbar_of(const Geo& g, const Nb& nb, int ci_, int cj
              (R(1) / S_(h, i, j, k)));
 }
 
-__global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__ u1,
+__global__ void __launch_bounds__(UBX * UBY) k_adj_upwind(Geo g, const R* __restrict__ u1,
                                                     const R* __restrict__ u2,
                                                     const R* __restrict__ u3,
                                                     const R* __restrict__ xb,
@@ -784,16 +786,38 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
                                                     const R* __restrict__ h, R* __restrict__ xbar,
                                                     R* __restrict__ ub1, R* __restrict__ ub2,
                                                     R* __restrict__ ub3, Adj a) {
+    // sbar of the block's cells and a one-cell ring on plane k, formed once
+    // each (the same expression as below, so the same bits) and read by the
+    // x and y faces of both neighbours; the z faces form sbar of planes k-1
+    // and k+1 themselves
+    constexpr int SX = UBX + 2, SY = UBY + 2;
+    __shared__ R sbl[SX * SY];
+    {
+        const int i0 = 1 + blockIdx.x * UBX, j0 = 1 + blockIdx.y * UBY, kk = 1 + blockIdx.z;
+        for (int q = threadIdx.y * UBX + threadIdx.x; q < SX * SY; q += UBX * UBY) {
+            const int ii = i0 - 1 + q % SX, jj = j0 - 1 + q / SX;
+            R v = R(0);
+            if (ii >= 1 && ii <= g.np && jj >= 1 && jj <= g.mp) {
+                const Nb nb = nb_at(g, ii, jj, kk);
+                v = sbar_of(g, nb, ii, jj, kk, a, rhr, h, ii, jj, kk);
+            }
+            sbl[q] = v;
+        }
+        __syncthreads();
+    }
     CELL_HEAD
     const int np = g.np, mp = g.mp, lp = g.lp;
+    // sbar of in-plane cell (ii, jj) from the block's LDS tile
+    const int sb0 = (threadIdx.y + 1) * SX + threadIdx.x + 1;
+#define SBL(di, dj) sbl[sb0 + (di) + (dj) * SX]
     const R w = S_(rhr, i, j, k) * psi_in(g, nb, ci_, cj_, ck_, a.psi, i, j, k);
     const R sc = -(w * (R(1) / S_(h, i, j, k)));
     R xb_c = w + x0_in(g, nb, ci_, cj_, ck_, a.x0, i, j, k);
     const R xc = S_(xb, i, j, k);
     // x faces i (c downstream) and i+1 (c upstream)
     {
-        const R fl = (i > 1 ? sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i - 1, j, k) : R(0)) - sc;
-        const R fh = sc - (i < np ? sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i + 1, j, k) : R(0));
+        const R fl = (i > 1 ? SBL(-1, 0) : R(0)) - sc;
+        const R fh = sc - (i < np ? SBL(1, 0) : R(0));
         const R ul = X_(u1, i, j, k), uh = X_(u1, i + 1, j, k);
         xb_c += fl * (ul < R(0) ? ul : R(0)) + fh * (uh > R(0) ? uh : R(0));
         X_(ub1, i, j, k) += a.uf * (fl * (ind_pos(ul) * S_(xb, i - 1, j, k) + ind_neg(ul) * xc));
@@ -801,8 +825,8 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
             X_(ub1, i + 1, j, k) += a.uf * (fh * (ind_pos(uh) * xc + ind_neg(uh) * S_(xb, i + 1, j, k)));
     }
     {
-        const R fl = (j > 1 ? sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i, j - 1, k) : R(0)) - sc;
-        const R fh = sc - (j < mp ? sbar_of(g, nb, ci_, cj_, ck_, a, rhr, h, i, j + 1, k) : R(0));
+        const R fl = (j > 1 ? SBL(0, -1) : R(0)) - sc;
+        const R fh = sc - (j < mp ? SBL(0, 1) : R(0));
         const R ul = Y_(u2, i, j, k), uh = Y_(u2, i, j + 1, k);
         xb_c += fl * (ul < R(0) ? ul : R(0)) + fh * (uh > R(0) ? uh : R(0));
         Y_(ub2, i, j, k) += a.uf * (fl * (ind_pos(ul) * S_(xb, i, j - 1, k) + ind_neg(ul) * xc));
@@ -836,6 +860,7 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
     if (i == np) { S_(a.x0, np + 1, j, k) = R(0); S_(a.psi_next, np + 1, j, k) = R(0); }
     if (j == 1) { S_(a.x0, i, 0, k) = R(0); S_(a.psi_next, i, 0, k) = R(0); }
     if (j == mp) { S_(a.x0, i, mp + 1, k) = R(0); S_(a.psi_next, i, mp + 1, k) = R(0); }
+#undef SBL
 }
 
 #undef CELL_HEAD
@@ -848,8 +873,8 @@ __global__ void __launch_bounds__(256) k_adj_upwind(Geo g, const R* __restrict__
 #undef YP
 #undef ZP
 
-inline dim3 cblock() { return dim3(64, 4, 1); }
-inline dim3 cgrid(const Geo& g) { return dim3((g.np + 63) / 64, (g.mp + 3) / 4, g.lp); }
+inline dim3 cblock() { return dim3(UBX, UBY, 1); }
+inline dim3 cgrid(const Geo& g) { return dim3((g.np + UBX - 1) / UBX, (g.mp + UBY - 1) / UBY, g.lp); }
 
 int check_adj(mpdata_ctx* c) {
     if (!c) return fail(MPDATA_EINVAL, "null context");
