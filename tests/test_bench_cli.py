@@ -33,3 +33,19 @@ def test_no_gpu_fails_without_a_result():
     r = _run("--steps", "1", "--warmup", "0", "--cpu-steps", "0")
     assert r.returncode != 0
     assert "{" not in r.stdout
+
+
+def test_ensemble_cpu_baseline_runs_uneven_members_on_the_oracle():
+    """The ensemble's CPU leg runs members whose planes are not square
+    (np > mp + 1 among them) through the C oracle; the reference build is
+    kept to square planes, where its y-wall zeroing stays in bounds."""
+    sys.path.insert(0, ROOT)
+    import bench
+    from advectionhpctester_amd import testdata
+    shapes = [(48, 16, 8), (16, 32, 8), (24, 24, 8)]
+    mem = [testdata.ensemble_member(*sh, seed=1000 + q) for q, sh in enumerate(shapes)]
+    x0 = [m["x"] for m in mem]
+    rest = [[m[k] for m in mem] for k in ("u1", "u2", "u3", "rhr", "h", "bcx", "bcy")]
+    out = bench.ensemble_cpu_baseline(x0, rest, shapes, max_cells=10 ** 9)
+    assert out["kind"] == "port" and out["cores"] == 1 and out["value"] > 0
+    assert "3 of the 3 members" in out["sample"]
