@@ -260,16 +260,19 @@ int use_fused(mpdata_ctx* c, int variant, int iters, int nonos, bool* fused) {
     return MPDATA_OK;
 }
 
-// One fused step out of place: the gauge variant on the fast build under
-// MPDATA_PATH_FAST, else the exactly rounded one; the standard variant on its
-// own (exactly rounded) kernel.  Three iterations keep the second iterate and
-// the second pass's limited velocities in the context's scratch.
+// One fused step out of place: under MPDATA_PATH_FAST on the fast build of
+// the variant's kernel, else on the exactly rounded one.  Three iterations
+// keep the second iterate and the second pass's limited velocities in the
+// context's scratch.
 template <typename T>
 int fused_step(mpdata_ctx* c, int variant, int iters, int nonos, const T* x, T* xout,
                const T* u1, const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
                const T* bcy, hipStream_t s) {
     if (variant == MPDATA_STANDARD) {
-        const hipError_t e = FusedStd<T>::step(c->g, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s);
+        const hipError_t e =
+            c->path == MPDATA_PATH_FAST
+                ? FusedStdFast<T>::step(c->g, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s)
+                : FusedStd<T>::step(c->g, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s);
         if (e != hipSuccess) return hip_fail(e, "fused standard step");
         return MPDATA_OK;
     }

@@ -78,6 +78,16 @@ struct FusedStd {
                              const StdTape<T>& tp, hipStream_t s);
 };
 
+// The same standard step built for speed rather than bit-identity
+// (mpdata_fused_fast.hip): FMA contraction and Newton-refined reciprocal
+// division, within 1e-12 of FusedStd; same scope, layout and halo results.
+template <typename T>
+struct FusedStdFast {
+    static hipError_t step(const Geo& g, int nonos, const T* x, T* xout, const T* u1,
+                           const T* u2, const T* u3, const T* rhr, const T* h, const T* bcx,
+                           const T* bcy, hipStream_t s);
+};
+
 // Intermediates of a limited two-pass step (standard or gauge variant) that
 // the adjoint step reads: x
 // with the inflow values on its wall halo (xb, the upwind's input) and with

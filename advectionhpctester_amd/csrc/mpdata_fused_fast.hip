@@ -1,9 +1,11 @@
-// The fused gauge step of the fast build: the kernel templates of
-// mpdata_fused_impl.hpp with FAST = true (Newton-refined reciprocal
-// division), and this file compiled with -ffp-contract=fast so that products
-// and sums contract into FMAs.  Not bit-identical to the reference's
-// operation order; the exact build is mpdata_fused.hip.
+// The fused steps of the fast build: the kernel templates of
+// mpdata_fused_impl.hpp (gauge) and mpdata_fused_std_impl.hpp (standard)
+// with FAST = true (Newton-refined reciprocal division), and this file
+// compiled with -ffp-contract=fast so that products and sums contract into
+// FMAs.  Not bit-identical to the reference's operation order; the exact
+// builds are mpdata_fused.hip and mpdata_fused_std.hip.
 #include "mpdata_fused_impl.hpp"
+#include "mpdata_fused_std_impl.hpp"
 
 namespace mpdata {
 
@@ -48,8 +50,17 @@ hipError_t FusedBatchFast<T>::step(const void* table, int n, int nitem, int cfg,
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t FusedStdFast<T>::step(const Geo& g, int nonos, const T* x, T* xout, const T* u1,
+                                 const T* u2, const T* u3, const T* rhr, const T* h,
+                                 const T* bcx, const T* bcy, hipStream_t s) {
+    return std_step<T, true>(g, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s);
+}
+
 template struct FusedFast<double>;
 template struct FusedFast<float>;
+template struct FusedStdFast<double>;
+template struct FusedStdFast<float>;
 template struct FusedBatchFast<double>;
 template struct FusedBatchFast<float>;
 

@@ -436,7 +436,8 @@ def multifield_bench(steps, size=(256, 256, 128), nfields=8):
     All fields gauge (mpdata_alg_choice 1), out of place with two buffer
     sets: one ce_advection call per step against eight single-field steps;
     then the reference's default mix (alg_choice 0: five gauge, three
-    standard fields), in place."""
+    standard fields), in place; exactly rounded builds, then the same three
+    on the fast builds ("fast")."""
     import torch
 
     from advectionhpctester_amd import mpdata, testdata
@@ -481,6 +482,14 @@ def multifield_bench(steps, size=(256, 256, 128), nfields=8):
         ms_together = timed(together)
         ms_single = timed(one_by_one)
         ms_mixed = timed(mixed)
+        # the same on the fast builds of both variants
+        d.set_path(mpdata.PATH_FAST)
+        fast = {"gauge_together_ms_per_step": timed(together),
+                "gauge_one_by_one_ms_per_step": timed(one_by_one),
+                "ce_mix_in_place_ms_per_step": timed(mixed)}
+        fast["gauge_together_value"] = cells / (fast["gauge_together_ms_per_step"] / 1e3)
+        fast["speedup_together_vs_one_by_one"] = (fast["gauge_one_by_one_ms_per_step"] /
+                                                   fast["gauge_together_ms_per_step"])
     return {"grid": list(size), "fields": nfields, "steps": steps,
             "gauge_together_ms_per_step": ms_together,
             "gauge_together_value": cells / (ms_together / 1e3),
@@ -488,7 +497,8 @@ def multifield_bench(steps, size=(256, 256, 128), nfields=8):
             "speedup_together_vs_one_by_one": ms_single / ms_together,
             "ce_mix_in_place_ms_per_step": ms_mixed,
             "ce_mix": "alg_choice 0: th, qv, qc standard (fused standard kernel, one at a time), "
-                      "u, v, w, tht, pstr gauge (one batched launch)"}
+                      "u, v, w, tht, pstr gauge (one batched launch)",
+            "fast": fast}
 
 
 # Arrays one call of the density step with a fixed time step must move per
@@ -504,9 +514,11 @@ def configs_bench(steps):
       periodic: 64x64x32, periodic in all directions, standard scheme, 2
         iterations, limiter off, solid-body rotation about z at Courant 0.5,
         Gaussian sphere (testdata.periodic_rotation_case); the reference's
-        time loop entry (mpdata_c_timeloop) over 100 steps on the fused
-        standard kernel; stages_ms_per_step the same steps on the per-stage
-        kernels; 56 B/cell.
+        time loop entry (mpdata_c_timeloop) over 100 steps on the fast build
+        of the fused standard kernel (path "fast"), then on the exactly
+        rounded one (exact_ms_per_step), the two fields after the 102 steps
+        within 1e-12 of each other (checked); stages_ms_per_step the same
+        steps on the per-stage kernels; 56 B/cell.
       deformational: 256x256x128, time-varying deformational flow at Courant
         0.7 with the velocity predictor every step, gauge, 3 iterations,
         limiter on, slotted cylinders, FP64 and FP32.  A step is the one-pass
@@ -560,7 +572,21 @@ def configs_bench(steps):
         def periodic(n):
             d.timeloop(n, t["x"], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"],
                        variant=mpdata.STANDARD, iters=2, nonos=0)
-        ms = timed(periodic, 100)
+        # the fast build timed as the headline is, then the exactly rounded
+        # one from the same start; their fields after the 102 steps compared
+        x_start = t["x"].clone()
+        c1_ms, c1_end = {}, {}
+        for name, path in (("fast", mpdata.PATH_FAST), ("exact", mpdata.PATH_FUSED)):
+            d.set_path(path)
+            t["x"].copy_(x_start)
+            c1_ms[name] = timed(periodic, 100)
+            c1_end[name] = t["x"].clone()
+        c1_err = float((c1_end["fast"] - c1_end["exact"]).abs().max() /
+                       c1_end["exact"].abs().max())
+        if not c1_err <= 1e-12:
+            raise SystemExit(f"periodic standard: fast build differs by {c1_err:.3e} from the "
+                             "exact build after 102 steps (bound 1e-12)")
+        ms = c1_ms["fast"]
         d.set_path(mpdata.PATH_STAGES)
         ms_stages = timed(periodic, 100)
     orc = Oracle(np.float64)
@@ -572,7 +598,8 @@ def configs_bench(steps):
                  (1, 1, 1), dims + (1,))
     dt_cpu = time.perf_counter() - t0
     out["periodic_64x64x32_standard"] = {
-        "ms_per_step": ms, "value": cells1 / (ms / 1e3), "path": "fused",
+        "ms_per_step": ms, "value": cells1 / (ms / 1e3), "path": "fast",
+        "exact_ms_per_step": c1_ms["exact"], "max_rel_err_fast_vs_exact_102_steps": c1_err,
         "stages_ms_per_step": ms_stages, "roofline": roof(cells1, ms, 56),
         "cpu_baseline": {"value": cells1 * n_cpu / dt_cpu, "unit": "cell-updates/s", "cores": 1,
                          "kind": "port",

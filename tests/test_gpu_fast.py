@@ -105,8 +105,8 @@ def test_fast_path_fp32_against_fp64_oracle(mp, orc64):
 
 def test_fast_path_scope(mp, orc64):
     """Options outside the fused kernels' scope are refused, not run
-    elsewhere; the standard variant, which has no fast build, runs its
-    exactly rounded fused kernel."""
+    elsewhere; the standard variant runs its own fast build (within 1e-12 of
+    the oracle)."""
     from advectionhpctester_amd.testdata import random_case
     dims = (24, 20, 10, 1)
     with mp.Domain(*dims) as d:
@@ -126,4 +126,54 @@ def test_fast_path_scope(mp, orc64):
         xo = c["x"].copy()
         orc64.step(mp.STANDARD, 2, 1, xo, c["u1"], c["u2"], c["u3"], c["rhr"], c["h"], c["bcx"],
                    c["bcy"], (0, 0, 0), dims)
-        assert bits_equal(interior(to_host(t["x"])), interior(xo))
+        assert _rel(interior(to_host(t["x"])), interior(xo)) <= TOL64
+
+
+@pytest.mark.parametrize("size", [(45, 19, 7), (96, 33, 17)])
+@pytest.mark.parametrize("ibc,nonos", [((0, 0, 0), 1), ((0, 0, 0), 0), ((1, 1, 1), 0),
+                                       ((1, 0, 1), 1)])
+def test_fast_standard_within_1e12_of_oracle(mp, orc64, size, ibc, nonos):
+    """The fast build of the standard step (limiter on and off, open and
+    periodic boundaries) against the oracle's exactly rounded steps: 50
+    steps within 1e-12 normwise relative, interior and halo."""
+    from advectionhpctester_amd.testdata import random_case
+    dims = size + (1,)
+    c = random_case(*dims, seed=sum(size) + 3 * nonos + sum(ibc), positive=False)
+    want = c["x"].copy()
+    nsteps = 50
+    for _ in range(nsteps):
+        orc64.step(mp.STANDARD, 2, nonos, want, c["u1"], c["u2"], c["u3"], c["rhr"], c["h"],
+                   c["bcx"], c["bcy"], ibc, dims)
+    with mp.Domain(*dims, ibc=ibc) as d:
+        d.set_path(mp.PATH_FAST)
+        t = {k: to_dev(v) for k, v in c.items()}
+        print(f"fast standard {size} ibc={ibc} nonos={nonos} {nsteps} steps", flush=True)
+        d.timeloop(nsteps, t["x"], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], t["bcx"],
+                   t["bcy"], variant=mp.STANDARD, iters=2, nonos=nonos)
+        got = to_host(t["x"])
+    err = _rel(interior(got), interior(want))
+    print(f"  max normwise relative error {err:.3e}", flush=True)
+    assert err <= TOL64, err
+    assert _rel(got, want) <= TOL64
+
+
+def test_fast_standard_not_bit_identical_but_close(mp):
+    """256x256x64 (the 32x16 tiling with long chunks), 20 limited steps:
+    the fast standard step stays within 1e-12 of the exactly rounded fused
+    step and is a different rounding."""
+    from advectionhpctester_amd.testdata import random_case
+    dims = (256, 256, 64, 1)
+    c = random_case(*dims, seed=9, positive=False)
+    res = {}
+    with mp.Domain(*dims) as d:
+        t = {k: to_dev(v) for k, v in c.items()}
+        for name, path in (("exact", mp.PATH_FUSED), ("fast", mp.PATH_FAST)):
+            print(f"256x256x64 standard {name} 20 steps", flush=True)
+            d.set_path(path)
+            x = t["x"].clone()
+            d.timeloop(20, x, t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], t["bcx"], t["bcy"],
+                       variant=mp.STANDARD, iters=2, nonos=1)
+            res[name] = to_host(x)
+    err = _rel(interior(res["fast"]), interior(res["exact"]))
+    print(f"fast vs exact standard after 20 steps: {err:.3e}", flush=True)
+    assert 0.0 < err <= TOL64
