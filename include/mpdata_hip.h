@@ -57,11 +57,13 @@ extern "C" {
 #define MPDATA_PATH_AUTO 0   /* fused kernel when it supports the options */
 #define MPDATA_PATH_STAGES 1 /* one kernel per reference stage            */
 #define MPDATA_PATH_FUSED 2  /* single-pass fused kernel (error if unsupported) */
-/* The fused kernel's fast build (FMA contraction, Newton-refined reciprocal
- * division) for the single-domain step, step_oop and time loop; not
- * bit-identical to the reference (a few ulps per step, about 1e-15
- * normwise after 50 steps at 512x512x256); error if unsupported.  The other
- * entry points treat it as MPDATA_PATH_AUTO. */
+/* The fused kernels' fast builds (FMA contraction, Newton-refined reciprocal
+ * division; one for the gauge and one for the standard variant) for the
+ * single-domain step, step_oop, the time loop and CE_advection's fields;
+ * not bit-identical to the reference (a few ulps per step, of order 1e-14
+ * normwise after 50 steps at 512x512x256, held to 1e-12); error if
+ * unsupported.  The other entry points (stage calls, the density step, the
+ * adjoint's recording) treat it as MPDATA_PATH_AUTO. */
 #define MPDATA_PATH_FAST 3
 
 typedef struct mpdata_ctx mpdata_ctx;
