@@ -98,6 +98,12 @@ class Domain:
         return torch.zeros(self.shape(kind), dtype=self.dtype, device=self.device)
 
     def set_path(self, path):
+        """PATH_AUTO (the fused kernel where it covers the options, else the
+        per-stage kernels; the default), PATH_STAGES, PATH_FUSED (the exactly
+        rounded fused kernel, an error where it does not apply) or PATH_FAST
+        (the fast builds of the gauge and standard fused steps: FMA
+        contraction and Newton-refined reciprocals, within 1e-12 of the
+        exact ones)."""
         check(LIB.mpdata_ctx_set_path(self._ctx, int(path)))
 
     def scratch_bytes(self):
