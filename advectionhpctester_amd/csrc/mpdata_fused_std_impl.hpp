@@ -1,8 +1,7 @@
 // The fused MPDATA step of the standard antidiffusive variant: donor-cell
-// upwind, the standard
-// pseudo-velocities, the non-oscillatory limiter on the donor fluxes of the
-// pseudo-velocities, the sign-aware limited velocities and the flux-form
-// update of the reference's two-step GPUBC path
+// upwind, the standard pseudo-velocities, the non-oscillatory limiter on the
+// donor fluxes of the pseudo-velocities, the sign-aware limited velocities
+// and the flux-form update of the reference's two-step GPUBC path
 // (advection/src_algorithms/upwind3d-gpubc.f90:101-198,
 // antidiff3d-standard-gpubc.f90:72-459) in one kernel that reads x, u1, u2,
 // u3, h, rhr once and writes the new x once.  The limiter can be off
