@@ -177,3 +177,24 @@ def test_fast_standard_not_bit_identical_but_close(mp):
     err = _rel(interior(res["fast"]), interior(res["exact"]))
     print(f"fast vs exact standard after 20 steps: {err:.3e}", flush=True)
     assert 0.0 < err <= TOL64
+
+
+def test_fast_standard_fp32_against_fp64_oracle(mp, orc64):
+    """The fast standard step in FP32 (limiter on, open walls), 20 steps,
+    against the FP64 oracle: the single-precision error, not more."""
+    from advectionhpctester_amd.testdata import random_case
+    dims = (64, 40, 12, 1)
+    c = random_case(*dims, seed=21)
+    want = c["x"].copy()
+    for _ in range(20):
+        orc64.step(mp.STANDARD, 2, 1, want, c["u1"], c["u2"], c["u3"], c["rhr"], c["h"],
+                   c["bcx"], c["bcy"], (0, 0, 0), dims)
+    with mp.Domain(*dims, dtype=torch.float32) as d:
+        d.set_path(mp.PATH_FAST)
+        t = {k: to_dev(v, torch.float32) for k, v in c.items()}
+        d.timeloop(20, t["x"], t["u1"], t["u2"], t["u3"], t["rhr"], t["h"], t["bcx"], t["bcy"],
+                   variant=mp.STANDARD, iters=2, nonos=1)
+        got = to_host(t["x"])
+    err = _rel(interior(got), interior(want))
+    print(f"FP32 fast standard vs FP64 oracle after 20 steps: {err:.3e}", flush=True)
+    assert err < 1e-5
