@@ -29,23 +29,7 @@ template <typename T>
 hipError_t FusedDens<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
                               const T* u3, const T* h, const T* bcx, const T* bcy,
                               const DensIO<T>& d, hipStream_t s) {
-    const Off o = make_off(g);
-    // 32x32 tiles with 64-plane chunks whenever they give at least half the
-    // CUs a work item: the step is not bandwidth-bound, and these re-read
-    // the fewest inputs (measured at 256x256x128: 1.60e9 B per call against
-    // 2.28e9 B with 32x16 tiles and 16-plane chunks, in the same 1.12 ms);
-    // the single-domain rule otherwise and for the tuning overrides
-    int cfg = 0, cl = 64;
-    int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        ncu < 1)
-        ncu = 256;
-    if (fused_config() >= 0 || fused_chunk_env() > 0 || 2 * items_of<32, 32>(g, 64) < ncu)
-        choose_tiling<T>(g, cfg, cl);
-    if (cl > g.lp) cl = g.lp;
-    if (cfg == 0) return launch_dens<T, 32, 32, 4>(g, o, cl, x, xout, u1, u2, u3, h, bcx, bcy, d, s);
-    return launch_dens<T, 32, 16, 4>(g, o, cl, x, xout, u1, u2, u3, h, bcx, bcy, d, s);
+    return dens_step<T, false>(g, x, xout, u1, u2, u3, h, bcx, bcy, d, s);
 }
 
 template <typename T>

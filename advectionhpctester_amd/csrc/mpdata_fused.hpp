@@ -131,6 +131,16 @@ struct FusedDens {
                            const DensIO<T>& d, hipStream_t s);
 };
 
+// The same density step built for speed (mpdata_fused_fast.hip: FMA
+// contraction, Newton-refined reciprocal division), within 1e-12 of
+// FusedDens; same scope and outputs.
+template <typename T>
+struct FusedDensFast {
+    static hipError_t step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                           const T* u3, const T* h, const T* bcx, const T* bcy,
+                           const DensIO<T>& d, hipStream_t s);
+};
+
 // One member of a batched step: its grid and fields (x in, xo out).
 template <typename T>
 struct BatchMember {

@@ -1,5 +1,6 @@
 // The fused steps of the fast build: the kernel templates of
-// mpdata_fused_impl.hpp (gauge) and mpdata_fused_std_impl.hpp (standard)
+// mpdata_fused_impl.hpp (gauge, density) and mpdata_fused_std_impl.hpp
+// (standard)
 // with FAST = true (Newton-refined reciprocal division), and this file
 // compiled with -ffp-contract=fast so that products and sums contract into
 // FMAs.  Not bit-identical to the reference's operation order; the exact
@@ -57,8 +58,17 @@ hipError_t FusedStdFast<T>::step(const Geo& g, int nonos, const T* x, T* xout, c
     return std_step<T, true>(g, nonos, x, xout, u1, u2, u3, rhr, h, bcx, bcy, s);
 }
 
+template <typename T>
+hipError_t FusedDensFast<T>::step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2,
+                                  const T* u3, const T* h, const T* bcx, const T* bcy,
+                                  const DensIO<T>& d, hipStream_t s) {
+    return dens_step<T, true>(g, x, xout, u1, u2, u3, h, bcx, bcy, d, s);
+}
+
 template struct FusedFast<double>;
 template struct FusedFast<float>;
+template struct FusedDensFast<double>;
+template struct FusedDensFast<float>;
 template struct FusedStdFast<double>;
 template struct FusedStdFast<float>;
 template struct FusedBatchFast<double>;

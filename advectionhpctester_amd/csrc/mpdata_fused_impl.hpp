@@ -457,7 +457,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 f3p = donor(xc, xq3, u3q2);
                 f3 = donor(xq1, xc, u3q1);
             }
-            if constexpr (DENS) xa_new = (xc - (f1p - f1 + f2p - f2 + f3p - f3) / hq2);
+            if constexpr (DENS) xa_new = (xc - div_<FAST>(f1p - f1 + f2p - f2 + f3p - f3, hq2));
             else xa_new = rq2 * (xc - (f1p - f1 + f2p - f2 + f3p - f3) * hinv);
             }
             sxa[R0][pj][pi] = xa_new;
@@ -649,7 +649,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
             f3c = F3p;
             T* const xoq = pl(xo, SXY, Q);
             if constexpr (DENS) {
-                const T xnew = (XA(2, li, lj) - (F1p - F1 + F2p - F2 + F3p - F3) / hQ);
+                const T xnew = (XA(2, li, lj) - div_<FAST>(F1p - F1 + F2p - F2 + F3p - F3, hQ));
                 sto(xoq, so, xnew);
                 const T xi = xnew * hQ;
                 T* const xincq = pl(dio->xinc, SXY, Q);
@@ -658,7 +658,7 @@ __device__ __forceinline__ void fused_item(const Geo& g, const Off& o, int ntx, 
                 sto(pl(dio->pexr1, SXY, Q), so, xi * ldo(pl(dio->pexe, SXY, Q), so));
                 sto(pl(dio->pexr2, SXY, Q), so, xi);
                 sto(pl(dio->rhoadv, SXY, Q), so, rho_n);
-                sto(pl(dio->rhr, SXY, Q), so, rho_n / xi);
+                sto(pl(dio->rhr, SXY, Q), so, div_<FAST>(rho_n, xi));
             } else {
                 const T r = rQ;
                 if constexpr (FAST)  // r*(xant/r - s/h) = xant - r*s/h, to rounding
@@ -1084,7 +1084,7 @@ hipError_t launch_three(const Geo& g, const Off& o, int cfg, int clen, const T* 
 
 // The density mode of the fixed-position kernel (DensIO), with the work-item
 // order of k_fused_gauge2.
-template <typename T, int RX, int RY, int MINB>
+template <typename T, int RX, int RY, int MINB, bool FAST = false>
 __global__ void __launch_bounds__(RX * RY, MINB)
 k_fused_dens(Geo g, Off o, int ntx, int nty, int nchunk, int clen, const T* __restrict__ x,
              const T* __restrict__ u1, const T* __restrict__ u2, const T* __restrict__ u3,
@@ -1104,12 +1104,12 @@ k_fused_dens(Geo g, Off o, int ntx, int nty, int nchunk, int clen, const T* __re
         const int tile = xcd * tiles_per_xcd + tix;
         if (tile >= ntile) continue;
         const int k0 = 1 + chunk * clen, k1 = min(g.lp, k0 + clen - 1);
-        fused_item<T, RX, RY, true>(g, o, ntx, tile, k0, k1, x, u1, u2, u3, nullptr, h, bcx, bcy,
-                                    xo, L, &dio);
+        fused_item<T, RX, RY, true, FAST>(g, o, ntx, tile, k0, k1, x, u1, u2, u3, nullptr, h, bcx,
+                                          bcy, xo, L, &dio);
     }
 }
 
-template <typename T, int RX, int RY, int MINB>
+template <typename T, int RX, int RY, int MINB, bool FAST = false>
 hipError_t launch_dens(const Geo& g, const Off& o, int clen, const T* x, T* xout, const T* u1,
                        const T* u2, const T* u3, const T* h, const T* bcx, const T* bcy,
                        const DensIO<T>& d, hipStream_t s) {
@@ -1120,13 +1120,14 @@ hipError_t launch_dens(const Geo& g, const Off& o, int clen, const T* x, T* xout
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused_dens<T, RX, RY, MINB>,
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu,
+                                                         k_fused_dens<T, RX, RY, MINB, FAST>,
                                                          RX * RY, 0);
     if (e != hipSuccess) return e;
     const int grid = persistent_grid(nitem, (long long)(ntx * nty + NXCD - 1) / NXCD * nchunk, ncu,
                                      per_cu);
-    k_fused_dens<T, RX, RY, MINB><<<grid, RX * RY, 0, s>>>(g, o, ntx, nty, nchunk, clen, x, u1, u2,
-                                                           u3, h, bcx, bcy, xout, d);
+    k_fused_dens<T, RX, RY, MINB, FAST><<<grid, RX * RY, 0, s>>>(g, o, ntx, nty, nchunk, clen, x,
+                                                                 u1, u2, u3, h, bcx, bcy, xout, d);
     return hipGetLastError();
 }
 
@@ -1270,6 +1271,30 @@ Off make_off(const Geo& g) {
     o.yx = g.np + 2 * g.ih;
     o.yxy = o.yx * (g.mp + 1 + 2 * g.ih);
     return o;
+}
+
+// The density step on the build FAST selects.  32x32 tiles with 64-plane
+// chunks whenever they give at least half the CUs a work item: the step is
+// not bandwidth-bound, and these re-read the fewest inputs (measured at
+// 256x256x128: 1.60e9 B per call against 2.28e9 B with 32x16 tiles and
+// 16-plane chunks, in the same 1.12 ms); the single-domain rule otherwise
+// and for the tuning overrides.
+template <typename T, bool FAST>
+hipError_t dens_step(const Geo& g, const T* x, T* xout, const T* u1, const T* u2, const T* u3,
+                     const T* h, const T* bcx, const T* bcy, const DensIO<T>& d, hipStream_t s) {
+    const Off o = make_off(g);
+    int cfg = 0, cl = 64;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu < 1)
+        ncu = 256;
+    if (fused_config() >= 0 || fused_chunk_env() > 0 || 2 * items_of<32, 32>(g, 64) < ncu)
+        choose_tiling<T>(g, cfg, cl);
+    if (cl > g.lp) cl = g.lp;
+    if (cfg == 0)
+        return launch_dens<T, 32, 32, 4, FAST>(g, o, cl, x, xout, u1, u2, u3, h, bcx, bcy, d, s);
+    return launch_dens<T, 32, 16, 4, FAST>(g, o, cl, x, xout, u1, u2, u3, h, bcx, bcy, d, s);
 }
 
 template <int RX, int RY>

@@ -305,7 +305,11 @@ int rho_run(mpdata_ctx* c, int liner, T* u1, T* u2, T* u3, T* x, T* xinc, const 
             if ((rc = ensure(c, &S.f[d], FACEX + d, sizeof(T)))) return rc;
         T *xn = (T*)S.xnext, *m1 = (T*)S.f[0], *m2 = (T*)S.f[1], *m3 = (T*)S.f[2];
         const DensIO<T> dio{xinc, pexe, rhoadv, rhr, pexr1, pexr2, m1, m2, m3};
-        HIPCHK(FusedDens<T>::step(g, x, xn, u1, u2, u3, h, bcx, bcy, dio, s));
+        // the fast build under MPDATA_PATH_FAST, else the exactly rounded one
+        if (c->path == MPDATA_PATH_FAST)
+            HIPCHK(FusedDensFast<T>::step(g, x, xn, u1, u2, u3, h, bcx, bcy, dio, s));
+        else
+            HIPCHK(FusedDens<T>::step(g, x, xn, u1, u2, u3, h, bcx, bcy, dio, s));
         dim3 cgrid((g.np + 2 + 63) / 64, (g.mp + 2 + 3) / 4, g.lp);
         k_rho_copyback<T><<<cgrid, cell_block(), 0, s>>>(g, xn, m1, m2, m3, x, u1, u2, u3);
         HIPCHK(hipGetLastError());

@@ -731,7 +731,8 @@ def density_bench(steps, size=(256, 256, 128)):
     Fourier field, h = 1 + 0.05 (Fourier field - 1), pexe = 1, zero
     inflow).  Fixed time step: one call = extrapolation to n+1/2, A->C
     averaging, level rotation and the density MPDATA step, all queued
-    without a host read-back.  Then with substepping at Courant 1.6 (two
+    without a host read-back; the same on the fast build of the fused
+    density kernel (fixed_fast).  Then with substepping at Courant 1.6 (two
     substeps; each call reads the Courant maxima back to the host)."""
     import torch
 
@@ -749,7 +750,10 @@ def density_bench(steps, size=(256, 256, 128)):
                               f["rhr"], f["pexe"], f["pexr1"], f["pexr2"], f["bcx"], f["bcy"],
                               1.0, 1.0, 1.0, 1.0, crmaxlim=0.9, dtmax=1.0, **flags)
 
-        for mode, flags, scale in (("fixed", {}, 1.0), ("substep", {"lsubstepping": 1}, 2.0)):
+        for mode, flags, scale, path in (("fixed", {}, 1.0, mpdata.PATH_AUTO),
+                                         ("fixed_fast", {}, 1.0, mpdata.PATH_FAST),
+                                         ("substep", {"lsubstepping": 1}, 2.0, mpdata.PATH_AUTO)):
+            d.set_path(path)
             for name in ("ox", "oy", "oz"):
                 f[name][0] *= scale
                 f[name][2] *= scale

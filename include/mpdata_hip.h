@@ -58,11 +58,11 @@ extern "C" {
 #define MPDATA_PATH_STAGES 1 /* one kernel per reference stage            */
 #define MPDATA_PATH_FUSED 2  /* single-pass fused kernel (error if unsupported) */
 /* The fused kernels' fast builds (FMA contraction, Newton-refined reciprocal
- * division; one for the gauge and one for the standard variant) for the
- * single-domain step, step_oop, the time loop and CE_advection's fields;
- * not bit-identical to the reference (a few ulps per step, of order 1e-14
- * normwise after 50 steps at 512x512x256, held to 1e-12); error if
- * unsupported.  The other entry points (stage calls, the density step, the
+ * division; one each for the gauge, the standard and the density step) for
+ * the single-domain step, step_oop, the time loop, CE_advection's fields
+ * and the density step; not bit-identical to the reference (a few ulps per
+ * step, of order 1e-14 normwise after 50 steps at 512x512x256, held to
+ * 1e-12); error if unsupported.  The other entry points (stage calls, the
  * adjoint's recording) treat it as MPDATA_PATH_AUTO. */
 #define MPDATA_PATH_FAST 3
 

@@ -96,6 +96,47 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("flags,cmax,steps", [({}, 0.5, 3), ({"lsubstepping": 1}, 1.7, 3),
+                                              ({"ltimeadapt": 1}, 1.3, 3),
+                                              ({"lsubstepping": 1, "lvertsplit": 1}, 1.7, 2)])
+def test_rho_gauge_fast_within_1e12_of_oracle(mp, flags, cmax, steps):
+    """PATH_FAST: the density step on the fast build of the fused kernel
+    (FMA contraction, Newton-refined reciprocals), the predictor driver as
+    on the other paths.  Every array within 1e-12 of the oracle (max
+    difference over the array's largest magnitude), the driver's scalar
+    state equal; the step's own outputs differ from the oracle's in the last
+    bits (the comparison is not vacuous)."""
+    from oracle import Oracle
+    dims = (37, 29, 13, 1)
+    f = _case(dims, 7, cmax, np.float64)
+    orc = Oracle(np.float64)
+    want = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in f.items()}
+    st0 = dict(nsubsteps=1, nsubsteps_o=1, dtn=f["dt"], dtnold=f["dt"])
+    so, sg = dict(st0), dict(st0)
+    dtmax = 8 * f["dt"]
+    differs = False
+    with mp.Domain(*dims) as d:
+        d.set_path(mp.PATH_FAST)
+        g = {k: to_dev(v) for k, v in f.items() if isinstance(v, np.ndarray)}
+        for s in range(steps):
+            print(f"fast density step flags={flags} step={s}", flush=True)
+            so = orc.rho_gauge(want, flags, so, 0.9, dtmax, 0, dims)
+            d.rho_gauge_gpubc(sg, g["ox"], g["oy"], g["oz"], g["u"], g["v"], g["w"], g["u1"],
+                              g["u2"], g["u3"], g["x_in"], g["x_incomp"], g["h"], g["rhoadv"],
+                              g["rhr"], g["pexe"], g["pexr1"], g["pexr2"], g["bcx"], g["bcy"],
+                              f["dxi"], f["dyi"], f["dzi"], f["dt"], liner=0, crmaxlim=0.9,
+                              dtmax=dtmax, **flags)
+            for k in ("nsubsteps", "nsubsteps_o", "dtn", "dtnold"):
+                assert sg[k] == so[k], (s, k, sg[k], so[k])
+            for k in FIELDS:
+                got, ref = to_host(g[k]), want[k]
+                scale = float(np.abs(ref).max())
+                err = float(np.abs(got - ref).max()) / (scale if scale > 0 else 1.0)
+                assert err <= 1e-12, f"step {s}, {k}: {err:.3e}"
+                differs = differs or not _same_bits(got, ref)
+    assert differs
+
+
 @pytest.mark.parametrize("path", ["PATH_AUTO", "PATH_STAGES"])
 @pytest.mark.parametrize("tdt,npdt", DTYPES)
 @pytest.mark.parametrize("flags,cmax,liner,steps", CASES)
